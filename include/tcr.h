@@ -1,0 +1,147 @@
+/*
+ * tcr.h -- C ABI of libtcr_hip.so, the MI355X (gfx950) compact-hashing radius
+ * search.  Plain pointers and sizes only; every device pointer is HIP device
+ * memory; every call is asynchronous on the given HIP stream (hipStream_t
+ * passed as void*) unless its comment says it synchronises.
+ *
+ * Reference interfaces replaced (torchCompactRadius 0.5.5, paths relative to
+ * its repository; cpp/ = src/torchCompactRadius/cppSrc/, py/ = src/torchCompactRadius/):
+ *   tcr_compute_hash_indices   <- torch.ops.*.computeHashIndices
+ *                                 (cpp/hashing.h:38, cpp/cppWrapper.cpp:45)
+ *   tcr_count_neighbors        <- torch.ops.*.countNeighbors
+ *                                 (cpp/neighborhood.h:102-108, cpp/cppWrapper.cpp:39)
+ *   tcr_build_neighbor_list    <- torch.ops.*.buildNeighborList
+ *                                 (cpp/neighborhood.h:117-124, cpp/cppWrapper.cpp:40)
+ *   tcr_search_count/_fill     <- neighborSearch_cpp + searchNeighbors_cpp
+ *                                 (py/compactHashing/cppSearch.py:12-98,179-244),
+ *                                 i.e. the compact branch of radiusSearch
+ *                                 (py/neighborhood.py:283) below the Python API,
+ *                                 for one cloud or a batch of clouds
+ *   tcr_search_export          <- the CompactHashMap record returned with
+ *                                 returnStructure=True
+ *                                 (py/searchAlgorithms/compactSearch.py:125-143)
+ *
+ * Errors: every entry point returns TCR_OK (0) or a negative code; the
+ * message of the last failure on the calling thread is tcr_last_error().
+ * No C++ exception crosses this boundary.
+ */
+#ifndef TCR_H
+#define TCR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCR_OK 0
+#define TCR_ERR_ARG (-1)      /* invalid argument (shape, mode, dim, sizes)     */
+#define TCR_ERR_HIP (-2)      /* a HIP runtime call failed                      */
+#define TCR_ERR_DOMAIN (-3)   /* degenerate grid (e.g. zero cells in a periodic dim) */
+#define TCR_ERR_SPACE (-4)    /* workspace or output smaller than required      */
+
+/* Support modes, same meaning as the reference's mode strings
+ * (cpp/common.h:68-70, cpp/neighborhoodDynamic_cpu.cpp:77-88). */
+#define TCR_MODE_SYMMETRIC 0
+#define TCR_MODE_GATHER 1
+#define TCR_MODE_SCATTER 2
+#define TCR_MODE_SUPERSYMMETRIC 3
+
+/* One point cloud pair of a (batched) search.  Positions are row-major
+ * [n, dim] fp32; supports are [n] fp32 (may be absent when the mode does not
+ * read them).  Offsets index the concatenated batch buffers passed to the
+ * search calls: *_off count points, *_foff count floats. */
+typedef struct tcr_cloud {
+    int32_t dim;            /* 1, 2 or 3                                        */
+    int32_t mode;           /* TCR_MODE_*                                       */
+    uint32_t periodic_mask; /* bit d set: dimension d is periodic               */
+    int32_t search_range;   /* cell stencil half-width, >= 1 (reference: 1)     */
+    float domain_min[3];
+    float domain_max[3];
+    uint32_t hash_map_length;
+    int32_t self_search;    /* 1: the query cloud is the reference cloud        */
+    int64_t q_off, nq;      /* query points of this cloud in the batch          */
+    int64_t r_off, nr;      /* reference points of this cloud in the batch      */
+    int64_t q_foff, r_foff; /* float offsets of the position rows               */
+} tcr_cloud;
+
+/* Device bytes of workspace a search over these clouds needs; the same
+ * workspace must be passed unchanged to tcr_search_count, _fill and _export. */
+size_t tcr_search_workspace_size(const tcr_cloud *clouds, int32_t n_clouds);
+
+/* Structure build (periodic wraps, hMax, hCell, cell keys, stable radix sort,
+ * cell table, hash table) and the neighbour count pass, then the exclusive
+ * scan of the counts into offsets[nq_total + 1] (int64, concatenated over the
+ * batch).  Synchronises the stream once at the end and writes the total pair
+ * count of the batch to *total_pairs.  qsup/rsup may be NULL when no cloud's
+ * mode reads them; for self-search clouds rpos/rsup may be NULL. */
+int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds,
+                     const float *qpos, const float *qsup,
+                     const float *rpos, const float *rsup,
+                     void *workspace, size_t workspace_bytes,
+                     int64_t *offsets, int64_t *total_pairs, void *stream);
+
+/* Pair fill after tcr_search_count with the same arguments: row (may be NULL)
+ * gets the query index, col the original reference index, both local to
+ * their cloud, at offsets[q] .. offsets[q+1] for query q, in the reference's
+ * order (cell stencil order, then sorted reference order). */
+int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds,
+                    const float *qpos, const float *qsup,
+                    const float *rpos, const float *rsup,
+                    void *workspace, size_t workspace_bytes,
+                    const int64_t *offsets, int64_t *row, int64_t *col, void *stream);
+
+/* Copy one cloud's structure out of the workspace in the reference layout:
+ * sort_index int64[nr], sorted_pos fp32[nr, dim], sorted_sup fp32[nr] (NULL to
+ * skip), cell_table int32[ncells, 3] (lin, begin, count) ordered by hash
+ * bucket, hash_table int32[L, 2] (begin, count | -1, 0), num_cells int32[dim]
+ * and the scalars (device) hcell fp32[1], ncells int64[1]. */
+int tcr_search_export(const tcr_cloud *clouds, int32_t n_clouds, int32_t cloud,
+                      void *workspace, size_t workspace_bytes,
+                      int64_t *sort_index, float *sorted_pos, float *sorted_sup,
+                      int32_t *cell_table, int32_t *hash_table, int32_t *num_cells,
+                      float *hcell, int64_t *ncells, void *stream);
+
+/* computeHashIndices: hash[i] = hashIndexing(cells[i, :dim], L)
+ * (cpp/hashing.h:20-35).  cells int32[n, dim], hash int32[n]. */
+int tcr_compute_hash_indices(const int32_t *cells, int64_t n, int32_t dim,
+                             int64_t hash_map_length, int32_t *hash, void *stream);
+
+/* countNeighbors on the reference's own structure layout: hash_table
+ * int32[L,2], cell_table int32[C,3], num_cells int32[dim], sorted positions
+ * fp32[nr,dim], domain arrays fp32[dim], periodicity uint8[dim] (torch.bool).
+ * counts int32[nq].  q_min is accepted and unused, as in the reference. */
+int tcr_count_neighbors(const float *query_pos, const float *query_sup, int64_t nq,
+                        int32_t search_range,
+                        const float *sorted_pos, const float *sorted_sup, int64_t nr,
+                        const int32_t *hash_table, int64_t hash_map_length,
+                        const int32_t *num_cells, const int32_t *cell_table, int64_t n_cells,
+                        const float *q_min, double h_cell,
+                        const float *max_domain, const float *min_domain,
+                        const uint8_t *periodicity, int32_t dim, int32_t mode,
+                        int32_t *counts, void *stream);
+
+/* buildNeighborList: writes (i, j_sorted) int64 pairs at offsets[i] (int32
+ * offsets, as the reference), list_length entries in total. */
+int tcr_build_neighbor_list(const int32_t *counts, const int32_t *offsets, int64_t list_length,
+                            const float *query_pos, const float *query_sup, int64_t nq,
+                            int32_t search_range,
+                            const float *sorted_pos, const float *sorted_sup, int64_t nr,
+                            const int32_t *hash_table, int64_t hash_map_length,
+                            const int32_t *num_cells, const int32_t *cell_table, int64_t n_cells,
+                            const float *q_min, double h_cell,
+                            const float *max_domain, const float *min_domain,
+                            const uint8_t *periodicity, int32_t dim, int32_t mode,
+                            int64_t *list_i, int64_t *list_j, void *stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tcr_last_error(void);
+
+/* Library version string. */
+const char *tcr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCR_H */

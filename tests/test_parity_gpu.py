@@ -1,0 +1,185 @@
+"""GPU parity: the HIP search (through libtcr_hip.so) against the CPU oracle,
+bit-exact on indices (pair sets and order) for every mode, dimension and
+periodicity, the edge cases of the reference path, and the workloads."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch
+from torchcompactradius_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def gpu_search(q, hq, r=None, hr=None, *, mn=None, mx=None, periodic=None, mode="gather", L=4096, fmt="coo",
+               structure=False):
+    dom = None
+    if mn is not None:
+        dom = DomainDescription(torch.tensor(mn, dtype=torch.float32), torch.tensor(mx, dtype=torch.float32),
+                                torch.tensor(periodic, dtype=torch.bool), len(mn))
+    ref = None if r is None else PointCloud(_t(r), _t(hr))
+    out = radiusSearch(PointCloud(_t(q), _t(hq)), ref, mode=mode, domain=dom, hashMapLength=L,
+                       algorithm="compact", format=fmt, returnStructure=structure)
+    torch.cuda.synchronize()
+    return out
+
+
+def assert_same(res_gpu, res_orc, fmt="coo"):
+    if fmt == "coo":
+        row = res_gpu.row.cpu().numpy()
+        col = res_gpu.col.cpu().numpy()
+        assert res_gpu.row.dtype == torch.int64 and res_gpu.col.dtype == torch.int64
+        np.testing.assert_array_equal(row, res_orc.row)
+        np.testing.assert_array_equal(col, res_orc.col)
+    else:
+        np.testing.assert_array_equal(res_gpu.indices.cpu().numpy(), res_orc.col)
+        assert res_gpu.indptr.dtype == torch.int32 and res_gpu.rowEntries.dtype == torch.int64
+        np.testing.assert_array_equal(res_gpu.indptr.cpu().numpy(), res_orc.indptr)
+        np.testing.assert_array_equal(res_gpu.rowEntries.cpu().numpy(), res_orc.counts.astype(np.int64))
+
+
+def test_plain_2d_1024_workload():
+    w = synthetic.plain_2d_1024()
+    print("case: plain-2d-1024", flush=True)
+    g = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, mode=w.mode,
+                   L=w.hash_map_length, fmt="coo")
+    o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode=w.mode, hash_map_length=w.hash_map_length)
+    assert_same(g, o)
+    assert 30000 < o.col.shape[0] < 36000
+
+
+@pytest.mark.parametrize("n", [20, 40])
+def test_periodic_3d_reduced(n):
+    w = synthetic.periodic_3d(n=n)
+    print(f"case: periodic-3d n={n}", flush=True)
+    g = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, mode=w.mode,
+                   L=w.hash_map_length, fmt="csr")
+    o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode=w.mode, hash_map_length=w.hash_map_length, want_rows=False)
+    assert_same(g, o, "csr")
+
+
+@pytest.mark.slow
+def test_periodic_3d_1m_full():
+    w = synthetic.periodic_3d(n=100)
+    print("case: periodic-3d-1M", flush=True)
+    g = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, mode=w.mode,
+                   L=w.hash_map_length, fmt="csr")
+    o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode=w.mode, hash_map_length=w.hash_map_length, want_rows=False)
+    assert_same(g, o, "csr")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("pmask", [0, 1, 7])
+def test_modes_dims_periodicity(dim, mode, pmask):
+    rng = np.random.default_rng(1000 + dim * 100 + pmask * 7 + len(mode))
+    nq, nr = 700, 900
+    per = [bool((pmask >> d) & 1) for d in range(dim)]
+    q = rng.uniform(-1, 1, (nq, dim)).astype(np.float32)
+    r = rng.uniform(-1, 1, (nr, dim)).astype(np.float32)
+    hq = rng.uniform(0.05, 0.2, nq).astype(np.float32)
+    hr = rng.uniform(0.05, 0.2, nr).astype(np.float32)
+    mn, mx = np.full(dim, -1, np.float32), np.full(dim, 1, np.float32)
+    print(f"case: dim={dim} mode={mode} pmask={pmask}", flush=True)
+    for selfs in (False, True):
+        g = gpu_search(q, hq, None if selfs else r, None if selfs else hr, mn=mn, mx=mx, periodic=per, mode=mode,
+                       L=331)
+        o = oracle.radius_search(q, hq, None if selfs else r, None if selfs else hr, domain_min=mn, domain_max=mx,
+                                 periodic=per, mode=mode, hash_map_length=331)
+        assert_same(g, o)
+
+
+def test_domain_none_and_points_outside():
+    """domain=None means [0,1]^d non-periodic; points outside the domain get
+    unclamped cells whose linear indices can alias in-domain cells."""
+    rng = np.random.default_rng(5)
+    q = rng.uniform(-0.7, 1.8, (1500, 2)).astype(np.float32)
+    h = np.full(1500, 0.09, np.float32)
+    print("case: domain=None", flush=True)
+    for L in (1, 7, 4096):
+        g = gpu_search(q, h, L=L)
+        o = oracle.radius_search(q, h, hash_map_length=L)
+        assert_same(g, o)
+
+
+def test_wrap_boundary_values():
+    """Values that once-wrap onto max and twice-wrap onto min (the reference
+    wraps the reference set twice and the query set once)."""
+    lo, hi = np.float32(0.1), np.float32(0.7)
+    vals = []
+    for b in (lo, hi, lo - (hi - lo), hi + (hi - lo)):
+        vals.append(b + np.arange(-40, 41, dtype=np.float32) * np.spacing(b))
+    v = np.concatenate(vals).astype(np.float32)
+    rng = np.random.default_rng(9)
+    q = np.stack([v, rng.uniform(0.1, 0.7, v.shape[0]).astype(np.float32)], 1)
+    h = np.full(q.shape[0], 0.05, np.float32)
+    mn, mx = np.array([lo, lo], np.float32), np.array([hi, hi], np.float32)
+    print("case: wrap boundaries", flush=True)
+    g = gpu_search(q, h, mn=mn, mx=mx, periodic=[True, True], L=64)
+    o = oracle.radius_search(q, h, domain_min=mn, domain_max=mx, periodic=[True, True], hash_map_length=64)
+    assert_same(g, o)
+
+
+def test_small_periodic_grids_visit_cells_twice():
+    """Two or three cells in a periodic dim: the stencil reaches the same
+    cell more than once and the reference emits duplicate pairs."""
+    rng = np.random.default_rng(12)
+    q = rng.uniform(0, 1, (300, 2)).astype(np.float32)
+    h = np.full(300, 0.4, np.float32)
+    print("case: small periodic grid", flush=True)
+    g = gpu_search(q, h, mn=[0, 0], mx=[1, 1], periodic=[True, False], L=16)
+    o = oracle.radius_search(q, h, domain_min=[0, 0], domain_max=[1, 1], periodic=[True, False], hash_map_length=16)
+    assert_same(g, o)
+
+
+def test_empty_and_tiny():
+    rng = np.random.default_rng(4)
+    r = rng.uniform(-1, 1, (64, 3)).astype(np.float32)
+    hr = np.full(64, 0.3, np.float32)
+    print("case: empty query", flush=True)
+    g = gpu_search(np.zeros((0, 3), np.float32), np.zeros(0, np.float32), r, hr, mn=[-1] * 3, mx=[1] * 3,
+                   periodic=[False] * 3)
+    assert g.row.numel() == 0
+    print("case: single point", flush=True)
+    g = gpu_search(r[:1], hr[:1], mn=[-1] * 3, mx=[1] * 3, periodic=[True] * 3)
+    assert g.row.tolist() == [0] and g.col.tolist() == [0]
+
+
+def test_structure_matches_oracle():
+    w = synthetic.periodic_3d(n=24)
+    print("case: returnStructure", flush=True)
+    g, hm = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, mode=w.mode,
+                       L=4096, fmt="csr", structure=True)
+    o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode=w.mode, hash_map_length=4096, want_rows=False, want_structure=True)
+    np.testing.assert_array_equal(hm.sortIndex.cpu().numpy(), o.sort_index)
+    np.testing.assert_array_equal(hm.numCells.cpu().numpy(), o.num_cells)
+    assert np.float32(hm.hCell.item()) == np.float32(o.hcell)
+    ht = hm.hashTable.cpu().numpy()
+    np.testing.assert_array_equal(ht[:, 1], o.hash_table[:, 1])
+    np.testing.assert_array_equal(ht[:, 0] == -1, o.hash_table[:, 0] == -1)
+    ct = hm.sortedCellTable.cpu().numpy()
+    assert ct.shape == o.cell_table.shape
+    # same cells per bucket (order inside a bucket is free)
+    for b in np.nonzero(o.hash_table[:, 1])[0][:500]:
+        s0, n0 = o.hash_table[b]
+        s1, n1 = ht[b]
+        assert sorted(map(tuple, o.cell_table[s0:s0 + n0])) == sorted(map(tuple, ct[s1:s1 + n1]))
+
+
+def test_deterministic_repeat():
+    w = synthetic.periodic_3d(n=30)
+    print("case: determinism", flush=True)
+    a = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, L=1 << 16, fmt="csr")
+    b = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, L=1 << 16, fmt="csr")
+    assert torch.equal(a.indices, b.indices) and torch.equal(a.indptr, b.indptr)

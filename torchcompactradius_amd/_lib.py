@@ -1,0 +1,104 @@
+"""ctypes binding of libtcr_hip.so (the C ABI declared in include/tcr.h).
+
+The library is the only compute path of this package: if it is missing or
+cannot be loaded, every search raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtcr_hip.so")
+
+MODES = {"symmetric": 0, "gather": 1, "scatter": 2, "superSymmetric": 3}
+TCR_OK, TCR_ERR_ARG, TCR_ERR_HIP, TCR_ERR_DOMAIN, TCR_ERR_SPACE = 0, -1, -2, -3, -4
+
+
+class TcrCloud(ctypes.Structure):
+    """Mirror of ``tcr_cloud`` in include/tcr.h."""
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("periodic_mask", ctypes.c_uint32),
+        ("search_range", ctypes.c_int32),
+        ("domain_min", ctypes.c_float * 3),
+        ("domain_max", ctypes.c_float * 3),
+        ("hash_map_length", ctypes.c_uint32),
+        ("self_search", ctypes.c_int32),
+        ("q_off", ctypes.c_int64),
+        ("nq", ctypes.c_int64),
+        ("r_off", ctypes.c_int64),
+        ("nr", ctypes.c_int64),
+        ("q_foff", ctypes.c_int64),
+        ("r_foff", ctypes.c_int64),
+    ]
+
+
+class TcrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libtcr_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+# Every symbol include/tcr.h declares (checked by the ABI tests).
+EXPORTED = (
+    "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_export",
+    "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
+    "tcr_last_error", "tcr_version",
+)
+
+
+def load() -> ctypes.CDLL:
+    """Load libtcr_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python torchcompactradius_amd/_build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, u8p = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p
+    cl = ctypes.POINTER(TcrCloud)
+    L.tcr_last_error.restype = ctypes.c_char_p
+    L.tcr_last_error.argtypes = []
+    L.tcr_version.restype = ctypes.c_char_p
+    L.tcr_version.argtypes = []
+    L.tcr_search_workspace_size.restype = ctypes.c_size_t
+    L.tcr_search_workspace_size.argtypes = [cl, i32]
+    L.tcr_search_count.restype = ctypes.c_int
+    L.tcr_search_count.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp,
+                                   ctypes.POINTER(ctypes.c_int64), vp]
+    L.tcr_search_fill.restype = ctypes.c_int
+    L.tcr_search_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.tcr_search_export.restype = ctypes.c_int
+    L.tcr_search_export.argtypes = [cl, i32, i32, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tcr_compute_hash_indices.restype = ctypes.c_int
+    L.tcr_compute_hash_indices.argtypes = [vp, i64, i32, i64, vp, vp]
+    L.tcr_count_neighbors.restype = ctypes.c_int
+    L.tcr_count_neighbors.argtypes = [vp, vp, i64, i32, vp, vp, i64, vp, i64, vp, vp, i64, vp,
+                                      ctypes.c_double, vp, vp, u8p, i32, i32, vp, vp]
+    L.tcr_build_neighbor_list.restype = ctypes.c_int
+    L.tcr_build_neighbor_list.argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp, i64, vp, vp, i64, vp,
+                                          ctypes.c_double, vp, vp, u8p, i32, i32, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != TCR_OK:
+        msg = load().tcr_last_error().decode(errors="replace")
+        if rc == TCR_ERR_ARG:
+            raise ValueError(msg)
+        raise TcrError(rc, msg)
+
+
+def ptr(t) -> Optional[int]:
+    """Device pointer of a tensor (None stays None)."""
+    if t is None:
+        return None
+    return t.data_ptr()

@@ -1,0 +1,384 @@
+// C ABI of libtcr_hip.so (declared in include/tcr.h).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tcr_internal.h"
+
+namespace tcr {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int build_structure(const Batch &b, Work &w, hipStream_t s);
+int query_count_generic(const Batch &b, Work &w, hipStream_t s);
+int query_fill_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
+                       hipStream_t s);
+struct GridView;
+struct RefOpArgs;
+
+static int64_t tiles_of(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) {
+    if (n < 1 || !clouds) {
+        err = "need at least one cloud";
+        return false;
+    }
+    b = Batch();
+    b.B = n;
+    b.dev.resize(n);
+    b.rt_prefix.assign(n + 1, 0);
+    for (int32_t c = 0; c < n; ++c) {
+        const tcr_cloud &h = clouds[c];
+        if (h.dim < 1 || h.dim > 3) { err = "dim must be 1, 2 or 3 (cloud " + std::to_string(c) + ")"; return false; }
+        if (h.mode < 0 || h.mode > 3) { err = "invalid support mode " + std::to_string(h.mode); return false; }
+        if (h.search_range < 1) { err = "search_range must be >= 1"; return false; }
+        if (h.hash_map_length < 1) { err = "hashMapLength must be >= 1"; return false; }
+        if (h.nq < 0 || h.nr < 0) { err = "negative point count"; return false; }
+        if (h.periodic_mask >> h.dim) { err = "periodic mask names a dimension beyond dim"; return false; }
+        CloudDev &d = b.dev[c];
+        std::memset(&d, 0, sizeof(d));
+        d.h = h;
+        if (h.self_search) d.h.nr = h.nq;
+        b.max_dim = h.dim > b.max_dim ? h.dim : b.max_dim;
+        d.rt_off = b.rt_total;
+        d.rt_n = tiles_of(d.h.nr);
+        b.rt_total += d.rt_n;
+        b.rt_prefix[c + 1] = b.rt_total;
+        d.hb_off = b.hb_total;
+        b.hb_total += h.hash_map_length;
+        b.nq_total += h.nq;
+        b.nr_total += d.h.nr;
+        if (!h.self_search) b.any_nonself = true;
+        for (int k = 0; k < 3; ++k) d.nc[k] = 1;
+    }
+    if (b.nr_total >= (int64_t)1 << 31 || b.nq_total >= (int64_t)1 << 31) {
+        err = "batch exceeds 2^31 points";
+        return false;
+    }
+    if (b.hb_total >= (int64_t)1 << 32) {
+        err = "sum of hash map lengths exceeds 2^32";
+        return false;
+    }
+    return true;
+}
+
+size_t carve_work(const Batch &b, void *ws, Work &w) {
+    Carve cv(ws);
+    const int64_t B = b.B, nr = b.nr_total, nq = b.nq_total, hb = b.hb_total;
+    w.desc = cv.take<CloudDev>(B);
+    w.q_start = cv.take<int64_t>(B + 1);
+    w.r_start = cv.take<int64_t>(B + 1);
+    w.rt_start = cv.take<int64_t>(B + 1);
+    w.qt_start = cv.take<int64_t>(B + 1);
+    w.hb_start = cv.take<int64_t>(B + 1);
+    w.rwrap = cv.take<float4>(nr);
+    w.keys0 = cv.take<uint32_t>(nr);
+    w.keys1 = cv.take<uint32_t>(nr);
+    w.vals0 = cv.take<uint32_t>(nr);
+    w.vals1 = cv.take<uint32_t>(nr);
+    w.hist = cv.take<uint32_t>((size_t)kRadix * b.rt_total);
+    int64_t scan_max = (int64_t)kRadix * b.rt_total;
+    scan_max = scan_max > nr ? scan_max : nr;
+    scan_max = scan_max > nq ? scan_max : nq;
+    scan_max = scan_max > hb ? scan_max : hb;
+    w.scan_part = cv.take<uint64_t>(scan_partials_needed(scan_max));
+    w.sorted = cv.take<float4>(nr);
+    w.sort_index = cv.take<int32_t>(nr);
+    w.skey = cv.take<int32_t>(nr);
+    w.flags = cv.take<uint32_t>(nr);
+    w.cellpos = cv.take<uint32_t>(nr);
+    w.cell_begin = cv.take<uint32_t>(nr + 1);
+    w.cell_lin = cv.take<int32_t>(nr);
+    w.cell_hash = cv.take<uint32_t>(nr);
+    w.cell_cloud = cv.take<uint32_t>(nr);
+    w.bucket_cnt = cv.take<uint32_t>(hb);
+    w.bucket_begin = cv.take<uint32_t>(hb);
+    w.bucket_fill = cv.take<uint32_t>(hb);
+    w.cell_table = cv.take<int32_t>(3 * (size_t)nr);
+    w.hash_table = cv.take<int32_t>(2 * (size_t)hb);
+    w.counts = cv.take<int32_t>(nq);
+    w.misc = cv.take<uint32_t>(64);
+    return cv.off + 256;
+}
+
+struct Prepared {
+    Batch b;
+    Work w;
+    std::vector<int64_t> pre;  // host prefixes; alive until the caller's sync
+};
+
+// Validates, carves the workspace and uploads descriptors and prefixes.
+static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup, const float *rpos,
+                   const float *rsup, void *ws, size_t ws_bytes, hipStream_t s, Prepared &p, bool upload,
+                   bool check_data = true) {
+    std::string err;
+    if (!make_batch(clouds, n, p.b, err)) {
+        set_error(err);
+        return TCR_ERR_ARG;
+    }
+    const size_t need = carve_work(p.b, ws, p.w);
+    if (!ws || ws_bytes < need) {
+        set_error("workspace too small: need " + std::to_string(need) + " bytes");
+        return TCR_ERR_SPACE;
+    }
+    Batch &b = p.b;
+    p.pre.assign(5 * (size_t)(b.B + 1), 0);
+    int64_t *qs = p.pre.data(), *rs = qs + b.B + 1, *rts = rs + b.B + 1, *qts = rts + b.B + 1, *hbs = qts + b.B + 1;
+    for (int32_t c = 0; c < b.B; ++c) {
+        CloudDev &d = b.dev[c];
+        const tcr_cloud &h = clouds[c];
+        if (check_data && h.nq > 0 && !qpos) { set_error("query positions missing"); return TCR_ERR_ARG; }
+        d.qp = qpos ? qpos + h.q_foff : nullptr;
+        d.qs = qsup ? qsup + h.q_off : nullptr;
+        if (h.self_search) {
+            d.rp = d.qp;
+            d.rs = d.qs;
+        } else {
+            if (check_data && h.nr > 0 && !rpos) { set_error("reference positions missing"); return TCR_ERR_ARG; }
+            d.rp = rpos ? rpos + h.r_foff : nullptr;
+            d.rs = rsup ? rsup + h.r_off : nullptr;
+        }
+        const bool need_q = h.mode != TCR_MODE_SCATTER, need_r = h.mode != TCR_MODE_GATHER;
+        if (check_data && need_q && h.nq > 0 && !d.qs) { set_error("query supports required by this mode"); return TCR_ERR_ARG; }
+        if (check_data && need_r && d.h.nr > 0 && !d.rs) { set_error("reference supports required by this mode"); return TCR_ERR_ARG; }
+        qs[c + 1] = qs[c] + h.nq;
+        rs[c + 1] = rs[c] + d.h.nr;
+        rts[c + 1] = rts[c] + d.rt_n;
+        qts[c + 1] = qts[c] + d.qt_n;
+        hbs[c + 1] = hbs[c] + h.hash_map_length;
+    }
+    if (upload) {
+        TCR_HIP(hipMemcpyAsync(p.w.desc, b.dev.data(), sizeof(CloudDev) * b.B, hipMemcpyHostToDevice, s));
+        TCR_HIP(hipMemcpyAsync(p.w.q_start, qs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+        TCR_HIP(hipMemcpyAsync(p.w.r_start, rs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+        TCR_HIP(hipMemcpyAsync(p.w.rt_start, rts, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+        TCR_HIP(hipMemcpyAsync(p.w.qt_start, qts, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+        TCR_HIP(hipMemcpyAsync(p.w.hb_start, hbs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+    }
+    return TCR_OK;
+}
+
+}  // namespace tcr
+
+using namespace tcr;
+
+extern "C" {
+
+const char *tcr_last_error(void) { return g_last_error.c_str(); }
+const char *tcr_version(void) { return "torchcompactradius_amd 0.1.0 (gfx950)"; }
+
+size_t tcr_search_workspace_size(const tcr_cloud *clouds, int32_t n_clouds) {
+    Batch b;
+    std::string err;
+    if (!make_batch(clouds, n_clouds, b, err)) {
+        set_error(err);
+        return 0;
+    }
+    Work w;
+    return carve_work(b, nullptr, w);
+}
+
+int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
+                     const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
+                     int64_t *offsets, int64_t *total_pairs, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, true);
+    if (rc) return rc;
+    rc = build_structure(p.b, p.w, s);
+    if (rc) return rc;
+    rc = query_count_generic(p.b, p.w, s);
+    if (rc) return rc;
+    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
+    if (rc) return rc;
+    // one synchronisation: total pair count plus the per-cloud error flags
+    std::vector<CloudDev> dv(p.b.B);
+    int64_t total = 0;
+    TCR_HIP(hipMemcpyAsync(&total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(dv.data(), p.w.desc, sizeof(CloudDev) * p.b.B, hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    for (int32_t c = 0; c < p.b.B; ++c)
+        if (dv[c].err) {
+            set_error("cloud " + std::to_string(c) + ": a periodic dimension has no cells (support exceeds the domain)");
+            return TCR_ERR_DOMAIN;
+        }
+    if (total_pairs) *total_pairs = total;
+    return TCR_OK;
+}
+
+int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
+                    const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
+                    const int64_t *offsets, int64_t *row, int64_t *col, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false);
+    if (rc) return rc;
+    return query_fill_generic(p.b, p.w, offsets, row, col, s);
+}
+
+__global__ void k_export_sort_index(const int32_t *__restrict__ si, int64_t n, int64_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = si[i];
+}
+__global__ void k_export_sorted(const float4 *__restrict__ s4, int64_t n, int dim, float *__restrict__ pos,
+                                float *__restrict__ sup) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = s4[i];
+    const float x[3] = {v.x, v.y, v.z};
+    if (pos)
+        for (int d = 0; d < dim; ++d) pos[i * dim + d] = x[d];
+    if (sup) sup[i] = v.w;
+}
+__global__ void k_export_scalars(const CloudDev *__restrict__ d, int32_t *__restrict__ nc, float *__restrict__ hcell,
+                                 int64_t *__restrict__ ncells) {
+    if (threadIdx.x != 0) return;
+    if (nc)
+        for (int k = 0; k < d->h.dim; ++k) nc[k] = d->nc[k];
+    if (hcell) *hcell = d->hcell;
+    if (ncells) *ncells = d->ncells;
+}
+
+int tcr_search_export(const tcr_cloud *clouds, int32_t n_clouds, int32_t cloud, void *workspace,
+                      size_t workspace_bytes, int64_t *sort_index, float *sorted_pos, float *sorted_sup,
+                      int32_t *cell_table, int32_t *hash_table, int32_t *num_cells, float *hcell, int64_t *ncells,
+                      void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    int rc = prepare(clouds, n_clouds, nullptr, nullptr, nullptr, nullptr, workspace, workspace_bytes, s, p, false,
+                     false);
+    if (rc) return rc;
+    if (cloud < 0 || cloud >= n_clouds) {
+        set_error("cloud index out of range");
+        return TCR_ERR_ARG;
+    }
+    const CloudDev &hd = p.b.dev[cloud];
+    int64_t r0 = 0;
+    for (int32_t c = 0; c < cloud; ++c) r0 += p.b.dev[c].h.nr;
+    const int64_t nr = hd.h.nr;
+    const unsigned g = (unsigned)((nr + 255) / 256);
+    if (nr > 0) {
+        if (sort_index)
+            hipLaunchKernelGGL(k_export_sort_index, dim3(g), dim3(256), 0, s, (const int32_t *)p.w.sort_index + r0,
+                               nr, sort_index);
+        if (sorted_pos || sorted_sup)
+            hipLaunchKernelGGL(k_export_sorted, dim3(g), dim3(256), 0, s, (const float4 *)p.w.sorted + r0, nr,
+                               hd.h.dim, sorted_pos, sorted_sup);
+    }
+    if (hash_table)
+        TCR_HIP(hipMemcpyAsync(hash_table, p.w.hash_table + 2 * hd.hb_off,
+                               sizeof(int32_t) * 2 * (size_t)hd.h.hash_map_length, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_export_scalars, dim3(1), dim3(64), 0, s, (const CloudDev *)p.w.desc + cloud, num_cells,
+                       hcell, ncells);
+    TCR_LAUNCH_CHECK();
+    if (cell_table) {
+        // the cell count lives on the device; copy the cloud's rows after a sync
+        CloudDev dd;
+        TCR_HIP(hipMemcpyAsync(&dd, p.w.desc + cloud, sizeof(CloudDev), hipMemcpyDeviceToHost, s));
+        TCR_HIP(hipStreamSynchronize(s));
+        if (dd.ncells > 0)
+            TCR_HIP(hipMemcpyAsync(cell_table, p.w.cell_table + 3 * dd.cell_off, sizeof(int32_t) * 3 * dd.ncells,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    return TCR_OK;
+}
+
+int tcr_compute_hash_indices(const int32_t *cells, int64_t n, int32_t dim, int64_t hash_map_length, int32_t *hash,
+                             void *stream) {
+    if (dim < 1 || dim > 3 || hash_map_length < 1 || n < 0) {
+        set_error("computeHashIndices: dim must be 1..3, hashMapLength >= 1");
+        return TCR_ERR_ARG;
+    }
+    return hash_indices_launch(cells, n, dim, (uint32_t)hash_map_length, hash, (hipStream_t)stream);
+}
+
+// Host copies of the small domain arrays are needed for the GridView; the
+// reference op receives them as device tensors, so they are read back here
+// (d floats each) before the launch.
+static int refop_common(const float *query_pos, const float *query_sup, int64_t nq, int32_t search_range,
+                        const float *sorted_pos, const float *sorted_sup, int64_t nr, const int32_t *hash_table,
+                        int64_t hash_map_length, const int32_t *num_cells, const int32_t *cell_table,
+                        double h_cell, const float *max_domain, const float *min_domain,
+                        const uint8_t *periodicity, int32_t dim, int32_t mode, hipStream_t s, RefOpArgs &a) {
+    if (dim < 1 || dim > 3) { set_error("Unsupported dimension: " + std::to_string(dim)); return TCR_ERR_ARG; }
+    if (mode < 0 || mode > 3) { set_error("Invalid support mode"); return TCR_ERR_ARG; }
+    if (search_range < 1 || hash_map_length < 1 || nq < 0 || nr < 0) { set_error("invalid sizes"); return TCR_ERR_ARG; }
+    if ((mode != TCR_MODE_SCATTER && nq > 0 && !query_sup) || (mode != TCR_MODE_GATHER && nr > 0 && !sorted_sup)) {
+        set_error("supports required by this mode are not defined");
+        return TCR_ERR_ARG;
+    }
+    float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    int32_t nc[3] = {1, 1, 1};
+    uint8_t per[3] = {0, 0, 0};
+    TCR_HIP(hipMemcpyAsync(mn, min_domain, sizeof(float) * dim, hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(mx, max_domain, sizeof(float) * dim, hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(nc, num_cells, sizeof(int32_t) * dim, hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(per, periodicity, sizeof(uint8_t) * dim, hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    a.qpos = query_pos;
+    a.qsup = query_sup;
+    a.nq = nq;
+    a.spos = sorted_pos;
+    a.ssup = sorted_sup;
+    a.g.hash_table = hash_table;
+    a.g.cell_table = cell_table;
+    a.g.L = (uint32_t)hash_map_length;
+    a.g.pmask = 0;
+    for (int d = 0; d < 3; ++d) {
+        a.g.nc[d] = nc[d];
+        a.g.mn[d] = mn[d];
+        a.g.mx[d] = mx[d];
+        if (d < dim && per[d]) {
+            a.g.pmask |= 1u << d;
+            if (nc[d] <= 0 && nq > 0) { set_error("periodic dimension with no cells"); return TCR_ERR_DOMAIN; }
+        }
+    }
+    a.g.hcell = (float)h_cell;
+    a.g.sr = search_range;
+    a.g.mode = mode;
+    return TCR_OK;
+}
+
+int tcr_count_neighbors(const float *query_pos, const float *query_sup, int64_t nq, int32_t search_range,
+                        const float *sorted_pos, const float *sorted_sup, int64_t nr, const int32_t *hash_table,
+                        int64_t hash_map_length, const int32_t *num_cells, const int32_t *cell_table,
+                        int64_t n_cells, const float *q_min, double h_cell, const float *max_domain,
+                        const float *min_domain, const uint8_t *periodicity, int32_t dim, int32_t mode,
+                        int32_t *counts, void *stream) {
+    (void)q_min;
+    (void)n_cells;
+    hipStream_t s = (hipStream_t)stream;
+    RefOpArgs a{};
+    int rc = refop_common(query_pos, query_sup, nq, search_range, sorted_pos, sorted_sup, nr, hash_table,
+                          hash_map_length, num_cells, cell_table, h_cell, max_domain, min_domain, periodicity, dim,
+                          mode, s, a);
+    if (rc) return rc;
+    a.counts = counts;
+    return refop_launch(dim, false, a, s);
+}
+
+int tcr_build_neighbor_list(const int32_t *counts, const int32_t *offsets, int64_t list_length,
+                            const float *query_pos, const float *query_sup, int64_t nq, int32_t search_range,
+                            const float *sorted_pos, const float *sorted_sup, int64_t nr,
+                            const int32_t *hash_table, int64_t hash_map_length, const int32_t *num_cells,
+                            const int32_t *cell_table, int64_t n_cells, const float *q_min, double h_cell,
+                            const float *max_domain, const float *min_domain, const uint8_t *periodicity,
+                            int32_t dim, int32_t mode, int64_t *list_i, int64_t *list_j, void *stream) {
+    (void)counts;
+    (void)q_min;
+    (void)n_cells;
+    (void)list_length;
+    hipStream_t s = (hipStream_t)stream;
+    RefOpArgs a{};
+    int rc = refop_common(query_pos, query_sup, nq, search_range, sorted_pos, sorted_sup, nr, hash_table,
+                          hash_map_length, num_cells, cell_table, h_cell, max_domain, min_domain, periodicity, dim,
+                          mode, s, a);
+    if (rc) return rc;
+    a.offsets32 = offsets;
+    a.li = list_i;
+    a.lj = list_j;
+    return refop_launch(dim, true, a, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Host-side internals shared by the translation units of libtcr_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tcr_device.h"
+
+namespace tcr {
+
+constexpr int kSortThreads = 256;          // 4 wavefronts per sort tile
+constexpr int kSortItems = 16;             // keys per thread
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+// Error reporting (thread-local message; see tcr_last_error()).
+void set_error(const std::string &msg);
+#define TCR_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ::tcr::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return TCR_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+#define TCR_LAUNCH_CHECK()                                                         \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ::tcr::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return TCR_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+// Bump allocator over a caller workspace; layout is a pure function of the
+// clouds so that size query and use agree.
+struct Carve {
+    char *base;
+    size_t off = 0;
+    explicit Carve(void *b) : base((char *)b) {}
+    template <typename T>
+    T *take(size_t n) {
+        off = (off + 255) & ~size_t(255);
+        T *p = base ? (T *)(base + off) : nullptr;
+        off += sizeof(T) * (n ? n : 1);
+        return p;
+    }
+};
+
+// Batch geometry derived on the host from the tcr_cloud array.
+struct Batch {
+    int32_t B = 0;
+    int64_t nq_total = 0, nr_total = 0;   // points
+    int64_t rt_total = 0, qt_total = 0;   // sort tiles
+    int64_t hb_total = 0;                 // hash buckets
+    int64_t qf_total = 0, rf_total = 0;   // floats
+    bool any_nonself = false;
+    int max_dim = 1;
+    std::vector<CloudDev> dev;            // host copy of the descriptors
+    std::vector<int64_t> rt_prefix;       // [B+1]
+};
+
+// Workspace pointers.
+struct Work {
+    CloudDev *desc;
+    int64_t *q_start;     // [B+1] query point prefix
+    int64_t *r_start;     // [B+1] reference point prefix
+    int64_t *rt_start;    // [B+1] reference tile prefix
+    int64_t *qt_start;    // [B+1] query tile prefix
+    int64_t *hb_start;    // [B+1] bucket prefix
+    float4 *rwrap;        // [Nr] wrapped reference positions (+support in .w)
+    uint32_t *keys0, *keys1;
+    uint32_t *vals0, *vals1;
+    uint32_t *hist;       // [kRadix * tiles]
+    uint64_t *scan_part;  // scan partials
+    float4 *sorted;       // [Nr] sorted positions (+support)
+    int32_t *sort_index;  // [Nr] sorted slot -> local original index
+    int32_t *skey;        // [Nr] sorted linear cell index
+    uint32_t *flags;      // [Nr]
+    uint32_t *cellpos;    // [Nr]
+    uint32_t *cell_begin; // [Nr+1] first global slot of each cell
+    int32_t *cell_lin;    // [Nr]
+    uint32_t *cell_hash;  // [Nr]
+    uint32_t *cell_cloud; // [Nr]
+    uint32_t *bucket_cnt; // [HB]
+    uint32_t *bucket_begin;
+    uint32_t *bucket_fill;
+    int32_t *cell_table;  // [Nr*3] reference layout, grouped by bucket
+    int32_t *hash_table;  // [HB*2]
+    int32_t *counts;      // [Nq]
+    uint32_t *misc;       // small counters
+};
+
+// One cloud's grid and tables as the traversal reads them.
+struct GridView {
+    const int32_t *hash_table;  // [L,2] of this cloud
+    const int32_t *cell_table;  // [C,3] of this cloud
+    uint32_t L;
+    int32_t nc[3];
+    uint32_t pmask;
+    float mn[3], mx[3];
+    float hcell;
+    int32_t sr;
+    int32_t mode;
+};
+
+// Arguments of the reference-layout countNeighbors / buildNeighborList ops.
+struct RefOpArgs {
+    const float *qpos, *qsup;
+    int64_t nq;
+    GridView g;
+    const float *spos, *ssup;
+    const int32_t *offsets32;
+    int64_t *li, *lj;
+    int32_t *counts;
+};
+int refop_launch(int dim, bool fill, const RefOpArgs &a, hipStream_t s);
+int hash_indices_launch(const int32_t *cells, int64_t n, int dim, uint32_t L, int32_t *out, hipStream_t s);
+
+bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err);
+size_t carve_work(const Batch &b, void *ws, Work &w);
+
+// Generic device-wide exclusive scan: out[i] = sum(in[0..i)), and
+// out[n] = total when write_total.  Tin: uint32_t or int32_t;
+// Tout: uint32_t or int64_t.
+template <typename Tin, typename Tout>
+int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
+                   hipStream_t s);
+size_t scan_partials_needed(int64_t n);
+
+// Stable LSD radix sort of the reference keys of every cloud (segmented by
+// cloud); result in (keys0, vals0) after an even number of passes.
+int radix_sort_ref(const Batch &b, Work &w, int passes, hipStream_t s);
+
+}  // namespace tcr

@@ -1,0 +1,138 @@
+// Device-wide exclusive scan (reduce -> scan partials -> apply), wave64.
+// Used for the offsets scan of the counts (cppSearch.py:81-82 restated with
+// int64 offsets), radix histograms, cell flags and hash buckets.
+#include "tcr_internal.h"
+
+namespace tcr {
+
+// Inclusive wave64 scan of v.
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T n = __shfl_up(v, o, kWave);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// Exclusive scan of the kScanTile values in lds (in place); returns the block
+// total.  Each thread owns kScanItems consecutive values.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan_tile(T *lds, T *wave_tot) {
+    const int t = threadIdx.x;
+    T loc[kScanItems];
+    T run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        loc[k] = lds[t * kScanItems + k];
+        run += loc[k];
+    }
+    T incl = wave_inclusive_scan<T>(run);
+    const int wave = t / kWave, lane = t & (kWave - 1);
+    if (lane == kWave - 1) wave_tot[wave] = incl;
+    __syncthreads();
+    T wave_pre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / kWave; ++w) {
+        const T x = wave_tot[w];
+        if (w < wave) wave_pre += x;
+        total += x;
+    }
+    T ex = wave_pre + incl - run;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        lds[t * kScanItems + k] = ex;
+        ex += loc[k];
+    }
+    __syncthreads();
+    return total;
+}
+
+template <typename Tin>
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const Tin *__restrict__ in, int64_t n,
+                                                               uint64_t *__restrict__ part) {
+    __shared__ uint64_t red[kScanThreads / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint64_t s = 0;
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+        const int64_t i = base + k;
+        if (i < n) s += (uint64_t)(int64_t)in[i];
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Single block: exclusive scan of nb partials in place; part[nb] = total.
+__global__ void __launch_bounds__(kScanThreads) k_scan_partials(uint64_t *__restrict__ part, int64_t nb) {
+    __shared__ uint64_t lds[kScanTile];
+    __shared__ uint64_t wt[kScanThreads / kWave];
+    uint64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kScanTile) {
+        for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+            lds[k] = (base + k < nb) ? part[base + k] : 0;
+        __syncthreads();
+        const uint64_t tot = block_exclusive_scan_tile<uint64_t>(lds, wt);
+        for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+            if (base + k < nb) part[base + k] = lds[k] + carry;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[nb] = carry;
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const Tin *__restrict__ in, Tout *__restrict__ out,
+                                                              int64_t n, const uint64_t *__restrict__ part,
+                                                              int64_t nb, int write_total) {
+    __shared__ uint64_t lds[kScanTile];
+    __shared__ uint64_t wt[kScanThreads / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+        const int64_t i = base + k;
+        lds[k] = (i < n) ? (uint64_t)(int64_t)in[i] : 0;
+    }
+    __syncthreads();
+    block_exclusive_scan_tile<uint64_t>(lds, wt);
+    const uint64_t pre = part[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = (Tout)(lds[k] + pre);
+    }
+    if (write_total && blockIdx.x == 0 && threadIdx.x == 0) out[n] = (Tout)part[nb];
+}
+
+__global__ void k_write_zero_total(uint64_t *part) { part[0] = 0; }
+
+size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 2; }
+
+template <typename Tin, typename Tout>
+int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
+                   hipStream_t s) {
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        if (write_total) TCR_HIP(hipMemsetAsync(out, 0, sizeof(Tout), s));
+        return TCR_OK;
+    }
+    hipLaunchKernelGGL(k_scan_reduce<Tin>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
+    hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
+                       (const uint64_t *)partials, nb, write_total ? 1 : 0);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
+
+template int exclusive_scan<uint32_t, uint32_t>(const uint32_t *, uint32_t *, int64_t, bool, uint64_t *,
+                                                hipStream_t);
+template int exclusive_scan<int32_t, int64_t>(const int32_t *, int64_t *, int64_t, bool, uint64_t *,
+                                              hipStream_t);
+
+}  // namespace tcr

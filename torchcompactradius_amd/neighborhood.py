@@ -1,0 +1,178 @@
+"""Public search API: ``radiusSearch`` and ``radius``.
+
+Mirrors torchCompactRadius 0.5.5 ``src/torchCompactRadius/neighborhood.py``:
+``radiusSearch`` (:136-309) with the same arguments, assertions and result
+types; ``algorithm='compact'`` (:283) runs on libtcr_hip.so.  ``radius``
+(:313-365) implements the torch_cluster-compatible behaviour the reference
+documents (README.md:45) -- flipped ``(i, j)`` order, per-batch search with
+index offsets -- on top of the batched search.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Tuple, Union
+
+import torch
+
+from .search import CloudSpec, CompactHashMap, export_structure, periodic_mask, run_search, _host_floats
+from .util import DomainDescription, PointCloud, SparseCOO, SparseCSR
+
+ALGORITHMS = ("naive", "small", "compact", "mlm")
+
+
+def _domain(domain: Optional[DomainDescription], dim: int):
+    """Default domain [0,1]^d, non-periodic (neighborhood.py:190-197)."""
+    if domain is None:
+        return [0.0] * dim, [1.0] * dim, 0
+    return (_host_floats(domain.min, dim, "domain.min"), _host_floats(domain.max, dim, "domain.max"),
+            periodic_mask(domain.periodic, dim))
+
+
+def radiusSearch(
+        queryPointCloud: PointCloud,
+        referencePointCloud: Optional[PointCloud] = None,
+        supportOverride: Optional[float] = None,
+        mode: str = "gather",
+        domain: Optional[DomainDescription] = None,
+        hashMapLength=4096,
+        algorithm: str = "naive",
+        verbose: bool = False,
+        format: str = "coo",
+        returnStructure: bool = False,
+        useDenseMLM: bool = False,
+        hashMapLengthAlgorithm: str = "primes",
+) -> Union[SparseCOO, SparseCSR, Tuple[Union[SparseCOO, SparseCSR], CompactHashMap]]:
+    """Fixed-radius neighbour search (neighborhood.py:136-309).
+
+    Returns the (query, reference) pairs with ``dist < support`` for the
+    support mode: ``gather`` h_i, ``scatter`` h_j, ``symmetric``
+    (h_i + h_j) / 2, ``superSymmetric`` max(h_i, h_j).  Rows are query
+    indices, columns reference indices, row-sorted; within a row the order is
+    the reference compact path's (cell stencil order, then original index).
+
+    Only ``algorithm='compact'`` is computed by this package; every search
+    runs on the GPU.  ``supportOverride`` maps a scalar support onto every
+    point (the reference's fixed-support compact variant is not callable).
+    """
+    with torch.no_grad():
+        assert algorithm in ALGORITHMS, f"algorithm = {algorithm} not supported"
+        assert format in ["coo", "csr"], f"format = {format} not supported"
+        assert mode in ["symmetric", "scatter", "gather", "superSymmetric"], f"mode = {mode} not supported"
+        assert queryPointCloud is not None, f"referencePointCloud = {queryPointCloud} is None"
+        if supportOverride is not None and not isinstance(supportOverride, float):
+            raise ValueError(f"supportOverride = {supportOverride} must be a float")
+        dim = queryPointCloud.positions.shape[1]
+        if referencePointCloud is not None:
+            assert referencePointCloud.positions.shape[1] == dim, \
+                f"referencePositions.shape[1] = {referencePointCloud.positions.shape[1]} != dimensionality = {dim}"
+        if domain is not None:
+            assert domain.min.shape[0] == dim, f"domainMin.shape[0] = {domain.min.shape[0]} != dimensionality = {dim}"
+            assert domain.max.shape[0] == dim, f"domainMax.shape[0] = {domain.max.shape[0]} != dimensionality = {dim}"
+        if queryPointCloud.supports is not None:
+            assert queryPointCloud.supports.shape[0] == queryPointCloud.positions.shape[0], \
+                "support.shape[0] != queryPositions.shape[0]"
+        if referencePointCloud is not None and referencePointCloud.supports is not None:
+            assert referencePointCloud.supports.shape[0] == referencePointCloud.positions.shape[0], \
+                "support.shape[0] != referencePositions.shape[0]"
+        if algorithm != "compact":
+            raise NotImplementedError(
+                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use algorithm='compact'")
+
+        qpos = queryPointCloud.positions
+        qsup = queryPointCloud.supports
+        self_search = referencePointCloud is None
+        rpos = None if self_search else referencePointCloud.positions
+        rsup = None if self_search else referencePointCloud.supports
+        if supportOverride is not None:
+            qsup = torch.full((qpos.shape[0],), supportOverride, dtype=torch.float32, device=qpos.device)
+            if not self_search:
+                rsup = torch.full((rpos.shape[0],), supportOverride, dtype=torch.float32, device=rpos.device)
+        from .search import _f32
+        qpos, qsup = _f32(qpos, "queryPointCloud.positions"), _f32(qsup, "queryPointCloud.supports")
+        rpos, rsup = _f32(rpos, "referencePointCloud.positions"), _f32(rsup, "referencePointCloud.supports")
+        mn, mx, pmask = _domain(domain, dim)
+        if verbose:
+            print(f"radiusSearch(compact): nq={qpos.shape[0]} nr={qpos.shape[0] if rpos is None else rpos.shape[0]} "
+                  f"dim={dim} mode={mode} periodic_mask={pmask} hashMapLength={hashMapLength} format={format}")
+        spec = CloudSpec(qpos=qpos, qsup=qsup, rpos=rpos, rsup=rsup, domain_min=mn, domain_max=mx,
+                         periodic=pmask, mode=mode, hash_map_length=int(hashMapLength))
+        res = run_search([spec], want_rows=(format == "coo"))
+        nq = qpos.shape[0]
+        nr = nq if rpos is None else rpos.shape[0]
+        if format == "coo":
+            out: Union[SparseCOO, SparseCSR] = SparseCOO(res.row, res.col, nq, nr)
+        else:
+            out = SparseCSR(res.col, res.offsets.int(), res.offsets[1:] - res.offsets[:-1], nq, nr)
+        if returnStructure:
+            return out, export_structure(res, 0, spec)
+        return out
+
+
+def radius(queryPositions: torch.Tensor,
+           referencePositions: Optional[torch.Tensor],
+           support: Union[float, torch.Tensor, Tuple[torch.Tensor, torch.Tensor]],
+           batch_x: Optional[torch.Tensor] = None, batch_y: Optional[torch.Tensor] = None,
+           mode: str = "gather",
+           domain: Optional[DomainDescription] = None,
+           hashMapLength=4096,
+           algorithm: str = "compact",
+           verbose: bool = False,
+           returnStructure: bool = False):
+    """torch_cluster-style radius (neighborhood.py:313-365, README.md:45):
+    returns ``(i, j)`` with ``i`` indexing the reference set and ``j`` the
+    query set (flipped relative to radiusSearch).  With ``batch_x`` /
+    ``batch_y``, each batch id is searched separately -- all ids in one
+    batched launch -- and indices are offset into the concatenated sets."""
+    with torch.no_grad():
+        if isinstance(support, tuple):
+            qsup, rsup = support
+        elif isinstance(support, torch.Tensor):
+            qsup, rsup = support, support if referencePositions is None else None
+        else:
+            qsup = torch.full((queryPositions.shape[0],), float(support), device=queryPositions.device)
+            rsup = None if referencePositions is None else torch.full(
+                (referencePositions.shape[0],), float(support), device=referencePositions.device)
+        dim = queryPositions.shape[1]
+        mn, mx, pmask = _domain(domain, dim)
+        if batch_x is None and batch_y is None:
+            ref = None if referencePositions is None else PointCloud(referencePositions, rsup)
+            res = radiusSearch(PointCloud(queryPositions, qsup), ref, mode=mode, domain=domain,
+                               hashMapLength=hashMapLength, algorithm="compact", format="coo",
+                               returnStructure=returnStructure)
+            coo = res[0] if returnStructure else res
+            if returnStructure:
+                return coo.col, coo.row, res[1]
+            return coo.col, coo.row
+        ref_pos = queryPositions if referencePositions is None else referencePositions
+        ref_sup = qsup if referencePositions is None else rsup
+        bx = batch_x if batch_x is not None else torch.zeros(queryPositions.shape[0], dtype=torch.long,
+                                                             device=queryPositions.device)
+        by = batch_y if batch_y is not None else (bx if referencePositions is None else torch.zeros(
+            ref_pos.shape[0], dtype=torch.long, device=ref_pos.device))
+        ids = torch.unique(torch.cat([bx, by])).tolist()
+        specs, qidx, ridx = [], [], []
+        for b in ids:
+            qi = torch.nonzero(bx == b).reshape(-1)
+            ri = torch.nonzero(by == b).reshape(-1)
+            qidx.append(qi)
+            ridx.append(ri)
+            specs.append(CloudSpec(qpos=queryPositions[qi].float().contiguous(),
+                                   qsup=None if qsup is None else qsup[qi].float().contiguous(),
+                                   rpos=ref_pos[ri].float().contiguous(),
+                                   rsup=None if ref_sup is None else ref_sup[ri].float().contiguous(),
+                                   domain_min=mn, domain_max=mx, periodic=pmask, mode=mode,
+                                   hash_map_length=int(hashMapLength)))
+        res = run_search(specs, want_rows=True)
+        # rows/cols are local to each cloud; map them back to global indices
+        counts = res.offsets[1:] - res.offsets[:-1]
+        qglob = torch.cat(qidx)
+        rows_global = qglob[torch.repeat_interleave(torch.arange(qglob.shape[0], device=qglob.device), counts)]
+        cloud_of_pair = torch.repeat_interleave(
+            torch.arange(len(specs), device=qglob.device),
+            torch.tensor([int(s.qpos.shape[0]) for s in specs], device=qglob.device))
+        cloud_of_pair = torch.repeat_interleave(cloud_of_pair, counts)
+        r_base = torch.tensor([0] + [int(r.shape[0]) for r in ridx], device=qglob.device).cumsum(0)[:-1]
+        rglob_all = torch.cat(ridx)
+        cols_global = rglob_all[res.col + r_base[cloud_of_pair]]
+        if returnStructure:
+            return cols_global, rows_global, None
+        return cols_global, rows_global

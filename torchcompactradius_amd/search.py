@@ -1,0 +1,231 @@
+"""Host driver of the compact search: validates tensors, fills ``tcr_cloud``
+descriptors and runs count -> (one sync for the pair total) -> fill through
+libtcr_hip.so on the current HIP stream.
+
+Reference counterpart: neighborSearch_cpp + searchNeighbors_cpp
+(src/torchCompactRadius/compactHashing/cppSearch.py:12-98,179-244).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, NamedTuple, Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import MODES, TcrCloud, check, ptr
+
+
+class CompactHashMap(NamedTuple):
+    """Structure of one search, the counterpart of the reference's record
+    (src/torchCompactRadius/compactHashing/datastructure.py:8-31)."""
+    sortedPositions: torch.Tensor
+    referencePositions: torch.Tensor
+    hashTable: torch.Tensor
+    hashMapLength: int
+    sortedCellTable: torch.Tensor
+    numCells: torch.Tensor
+    qMin: torch.Tensor
+    qMax: torch.Tensor
+    minD: torch.Tensor
+    maxD: torch.Tensor
+    sortIndex: torch.Tensor
+    hCell: torch.Tensor
+    periodicity: torch.Tensor
+    searchRadius: int
+    sortedSupports: Optional[torch.Tensor] = None
+    referenceSupports: Optional[torch.Tensor] = None
+    fixedSupport: Optional[float] = None
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _require_device(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise RuntimeError(f"{name} is not on the GPU: torchcompactradius_amd runs its search only on "
+                           "HIP devices (no CPU path)")
+
+
+def _f32(t: Optional[torch.Tensor], name: str) -> Optional[torch.Tensor]:
+    if t is None:
+        return None
+    _require_device(t, name)
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32 (got {t.dtype})")
+    return t.contiguous()
+
+
+def periodic_mask(periodic, dim: int) -> int:
+    if periodic is None:
+        return 0
+    if isinstance(periodic, torch.Tensor):
+        periodic = periodic.tolist()
+    if isinstance(periodic, bool):
+        periodic = [periodic] * dim
+    m = 0
+    for d, p in enumerate(periodic):
+        if bool(p):
+            m |= 1 << d
+    return m
+
+
+def _host_floats(v, dim: int, name: str) -> List[float]:
+    if isinstance(v, torch.Tensor):
+        v = v.detach().to("cpu", torch.float32).reshape(-1).tolist()
+    v = [float(x) for x in v]
+    if len(v) != dim:
+        raise ValueError(f"{name} has {len(v)} entries, expected {dim}")
+    return v
+
+
+@dataclass
+class CloudSpec:
+    """One query/reference pair of a batched search."""
+    qpos: torch.Tensor
+    qsup: Optional[torch.Tensor]
+    rpos: Optional[torch.Tensor]        # None: self-search
+    rsup: Optional[torch.Tensor]
+    domain_min: Sequence[float]
+    domain_max: Sequence[float]
+    periodic: int                       # bit mask
+    mode: str = "gather"
+    hash_map_length: int = 4096
+    search_range: int = 1
+
+
+class SearchResult(NamedTuple):
+    offsets: torch.Tensor               # int64 [nq_total + 1]
+    row: Optional[torch.Tensor]         # int64 [P] (local query index) or None
+    col: torch.Tensor                   # int64 [P] (local reference index)
+    total: int
+    workspace: torch.Tensor
+    clouds: ctypes.Array
+
+
+def _cloud_array(specs: Sequence[CloudSpec], q_offs, r_offs) -> ctypes.Array:
+    arr = (TcrCloud * len(specs))()
+    for k, s in enumerate(specs):
+        c = arr[k]
+        dim = s.qpos.shape[1]
+        c.dim = dim
+        c.mode = MODES[s.mode]
+        c.periodic_mask = s.periodic
+        c.search_range = s.search_range
+        for d in range(dim):
+            c.domain_min[d] = s.domain_min[d]
+            c.domain_max[d] = s.domain_max[d]
+        c.hash_map_length = s.hash_map_length
+        c.self_search = 1 if s.rpos is None else 0
+        c.nq = s.qpos.shape[0]
+        c.q_off, c.q_foff = q_offs[k]
+        if s.rpos is None:
+            c.nr, c.r_off, c.r_foff = c.nq, 0, 0
+        else:
+            c.nr = s.rpos.shape[0]
+            c.r_off, c.r_foff = r_offs[k]
+    return arr
+
+
+def run_search(specs: Sequence[CloudSpec], want_rows: bool, *, qpos_cat=None, qsup_cat=None,
+               rpos_cat=None, rsup_cat=None, timers: Optional[dict] = None) -> SearchResult:
+    """Search a batch of clouds; inputs are concatenated into flat device
+    buffers unless already given (``*_cat``) with offsets implied by order.
+    ``timers`` (optional) receives torch.cuda.Event pairs recorded on the
+    search stream around the count and fill launches."""
+    L = _lib.load()
+    if len(specs) == 0:
+        raise ValueError("empty batch")
+    device = specs[0].qpos.device
+    q_offs, r_offs = [], []
+    qo = qf = ro = rf = 0
+    for s in specs:
+        q_offs.append((qo, qf))
+        qo += s.qpos.shape[0]
+        qf += s.qpos.numel()
+        if s.rpos is not None:
+            r_offs.append((ro, rf))
+            ro += s.rpos.shape[0]
+            rf += s.rpos.numel()
+        else:
+            r_offs.append((0, 0))
+    if qpos_cat is None:
+        if len(specs) == 1:
+            qpos_cat = specs[0].qpos.reshape(-1)
+            qsup_cat = specs[0].qsup
+            rpos_cat = None if specs[0].rpos is None else specs[0].rpos.reshape(-1)
+            rsup_cat = specs[0].rsup
+        else:
+            qpos_cat = torch.cat([s.qpos.reshape(-1) for s in specs])
+            qs = [s.qsup for s in specs]
+            qsup_cat = torch.cat([q if q is not None else torch.zeros(s.qpos.shape[0], device=device)
+                                  for q, s in zip(qs, specs)]) if any(q is not None for q in qs) else None
+            rl = [s for s in specs if s.rpos is not None]
+            rpos_cat = torch.cat([s.rpos.reshape(-1) for s in rl]) if rl else None
+            rsl = [s.rsup for s in rl]
+            rsup_cat = torch.cat([r if r is not None else torch.zeros(s.rpos.shape[0], device=device)
+                                  for r, s in zip(rsl, rl)]) if any(r is not None for r in rsl) else None
+    clouds = _cloud_array(specs, q_offs, r_offs)
+    n = len(specs)
+    ws_bytes = L.tcr_search_workspace_size(clouds, n)
+    if ws_bytes == 0:
+        check(_lib.TCR_ERR_ARG)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    offsets = torch.empty(qo + 1, dtype=torch.int64, device=device)
+    total = ctypes.c_int64(0)
+    stream = _stream()
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    t0 = ev() if timers is not None else None
+    check(L.tcr_search_count(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
+                             ptr(ws), ws_bytes, ptr(offsets), ctypes.byref(total), stream))
+    t1 = ev() if timers is not None else None
+    P = int(total.value)
+    col = torch.empty(P, dtype=torch.int64, device=device)
+    row = torch.empty(P, dtype=torch.int64, device=device) if want_rows else None
+    t2 = ev() if timers is not None else None
+    check(L.tcr_search_fill(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
+                            ptr(ws), ws_bytes, ptr(offsets), ptr(row), ptr(col), stream))
+    if timers is not None:
+        t3 = ev()
+        timers.setdefault("count", []).append((t0, t1))
+        timers.setdefault("fill", []).append((t2, t3))
+    return SearchResult(offsets, row, col, P, ws, clouds)
+
+
+def export_structure(res: SearchResult, cloud: int, spec: CloudSpec) -> CompactHashMap:
+    """Copy one cloud's structure out of the workspace (returnStructure)."""
+    L = _lib.load()
+    c = res.clouds[cloud]
+    dim, nr, Lh = c.dim, c.nr, c.hash_map_length
+    dev = res.workspace.device
+    sort_index = torch.empty(nr, dtype=torch.int64, device=dev)
+    sorted_pos = torch.empty((nr, dim), dtype=torch.float32, device=dev)
+    sorted_sup = torch.empty(nr, dtype=torch.float32, device=dev)
+    hash_table = torch.empty((Lh, 2), dtype=torch.int32, device=dev)
+    num_cells = torch.empty(dim, dtype=torch.int32, device=dev)
+    hcell = torch.empty((), dtype=torch.float32, device=dev)
+    ncells = torch.empty(1, dtype=torch.int64, device=dev)
+    cell_table = torch.empty((max(nr, 1), 3), dtype=torch.int32, device=dev)
+    check(L.tcr_search_export(res.clouds, len(res.clouds), cloud, ptr(res.workspace), res.workspace.numel(),
+                              ptr(sort_index), ptr(sorted_pos), ptr(sorted_sup), ptr(cell_table), ptr(hash_table),
+                              ptr(num_cells), ptr(hcell), ptr(ncells), _stream()))
+    nc = int(ncells.item())
+    mn = torch.tensor(list(spec.domain_min), dtype=torch.float32, device=dev)
+    mx = torch.tensor(list(spec.domain_max), dtype=torch.float32, device=dev)
+    per = torch.tensor([bool((spec.periodic >> d) & 1) for d in range(dim)], dtype=torch.bool, device=dev)
+    has_sup = (spec.rsup if spec.rpos is not None else spec.qsup) is not None
+    return CompactHashMap(
+        sortedPositions=sorted_pos, referencePositions=spec.rpos if spec.rpos is not None else spec.qpos,
+        hashTable=hash_table, hashMapLength=Lh, sortedCellTable=cell_table[:nc], numCells=num_cells,
+        qMin=mn, qMax=mx, minD=mn, maxD=mx, sortIndex=sort_index, hCell=hcell, periodicity=per,
+        searchRadius=spec.search_range, sortedSupports=sorted_sup if has_sup else None,
+        referenceSupports=spec.rsup if spec.rpos is not None else spec.qsup)

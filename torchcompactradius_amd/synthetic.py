@@ -1,0 +1,75 @@
+"""Seeded synthetic point clouds for the benchmark and parity workloads.
+
+All clouds are generated on the host in float64 with
+``numpy.random.default_rng(seed)`` and cast to float32, so the GPU search and
+the CPU oracle receive identical bits.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    pos: np.ndarray                 # float32 [N, d]
+    sup: np.ndarray                 # float32 [N]
+    domain_min: np.ndarray          # float32 [d]
+    domain_max: np.ndarray          # float32 [d]
+    periodic: List[bool]
+    mode: str
+    hash_map_length: int
+    format: str
+    rpos: Optional[np.ndarray] = None   # None: self-search
+    rsup: Optional[np.ndarray] = None
+    meta: dict = field(default_factory=dict)
+
+
+def jittered_grid(n_per_dim: int, dim: int, lo: float, hi: float, jitter_frac: float,
+                  rng: np.random.Generator) -> np.ndarray:
+    """Cell centres of an n^dim grid over [lo, hi]^dim, each jittered by
+    U(+-jitter_frac * dx); index order is row-major with dim 0 slowest."""
+    dx = (hi - lo) / n_per_dim
+    g = lo + (np.arange(n_per_dim) + 0.5) * dx
+    mesh = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), axis=-1).reshape(-1, dim)
+    mesh = mesh + rng.uniform(-jitter_frac * dx, jitter_frac * dx, size=mesh.shape)
+    return mesh.astype(np.float32)
+
+
+def support_for(neighbors: float, dx: float, dim: int) -> float:
+    if dim == 1:
+        return neighbors * dx / 2
+    if dim == 2:
+        return math.sqrt(neighbors * dx * dx / math.pi)
+    return (3.0 * neighbors * dx ** 3 / (4.0 * math.pi)) ** (1.0 / 3.0)
+
+
+def plain_2d_1024(seed: int = 1) -> Workload:
+    """2-D 32x32 jittered grid in [-1,1]^2 (1024 points), ~32 neighbours,
+    non-periodic, gather, hashMapLength 4096, COO."""
+    rng = np.random.default_rng(seed)
+    n, dim = 32, 2
+    dx = 2.0 / n
+    pos = jittered_grid(n, dim, -1.0, 1.0, 0.05, rng)
+    h = np.float32(math.sqrt(32 * dx * dx / math.pi))
+    return Workload("plain-2d-1024", pos, np.full(pos.shape[0], h, np.float32),
+                    np.full(dim, -1, np.float32), np.full(dim, 1, np.float32), [False] * dim,
+                    "gather", 4096, "coo", meta={"dx": dx, "h": float(h)})
+
+
+def periodic_3d(n: int = 100, seed: int = 2, neighbors: float = 50.0) -> Workload:
+    """3-D n^3 jittered grid in [-1,1]^3 (n=100: 1M points), jitter
+    +-0.1 dx, uniform h for ~50 neighbours, periodic in every dim, gather,
+    hashMapLength 2^20, CSR."""
+    rng = np.random.default_rng(seed)
+    dim = 3
+    dx = 2.0 / n
+    pos = jittered_grid(n, dim, -1.0, 1.0, 0.1, rng)
+    h = np.float32(support_for(neighbors, dx, dim))
+    return Workload(f"periodic-3d-{n**3}", pos, np.full(pos.shape[0], h, np.float32),
+                    np.full(dim, -1, np.float32), np.full(dim, 1, np.float32), [True] * dim,
+                    "gather", 1 << 20, "csr", meta={"dx": dx, "h": float(h), "n": n})
