@@ -24,7 +24,8 @@ __global__ void __launch_bounds__(kBlock) k_hmax(CloudDev *__restrict__ desc, in
         c = cloud_of(B, g, start);
         const CloudDev &cd = desc[c];
         const int mode = cd.h.mode;
-        const bool use = which == 0 ? (mode != TCR_MODE_SCATTER)
+        // a self-search cloud's query set is its reference set
+        const bool use = which == 0 ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
                                     : (mode != TCR_MODE_GATHER && !cd.h.self_search);
         const float *sup = which == 0 ? cd.qs : cd.rs;
         if (use && sup) v = float_to_ordered(sup[g - start[c]]);
