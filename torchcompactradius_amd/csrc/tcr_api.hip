@@ -11,8 +11,8 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 int build_structure(const Batch &b, Work &w, hipStream_t s);
-int query_count_generic(const Batch &b, Work &w, hipStream_t s);
-int query_fill_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
+int query_count(const Batch &b, Work &w, hipStream_t s);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
                        hipStream_t s);
 struct GridView;
 struct RefOpArgs;
@@ -99,6 +99,10 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
     w.hash_table = cv.take<int32_t>(2 * (size_t)hb);
     w.counts = cv.take<int32_t>(nq);
     w.misc = cv.take<uint32_t>(64);
+    w.pflags = cv.take<uint8_t>(nr);
+    w.sflags = cv.take<uint8_t>(nr);
+    w.cursor = cv.take<int32_t>(nq);
+    w.stray_list = cv.take<int32_t>(nq);
     return cv.off + 256;
 }
 
@@ -188,7 +192,7 @@ int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds, const float *qpo
     if (rc) return rc;
     rc = build_structure(p.b, p.w, s);
     if (rc) return rc;
-    rc = query_count_generic(p.b, p.w, s);
+    rc = query_count(p.b, p.w, s);
     if (rc) return rc;
     rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
     if (rc) return rc;
@@ -214,7 +218,7 @@ int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos
     Prepared p;
     int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false);
     if (rc) return rc;
-    return query_fill_generic(p.b, p.w, offsets, row, col, s);
+    return query_fill(p.b, p.w, offsets, row, col, s);
 }
 
 __global__ void k_export_sort_index(const int32_t *__restrict__ si, int64_t n, int64_t *__restrict__ out) {

@@ -2,6 +2,8 @@
 // positions, cell table and compact hash table -- the GPU form of
 // neighborSearch_cpp's preamble (py/compactHashing/cppSearch.py:215-224) and
 // buildCompactHashMap_compat (py/compactHashing/hashTable.py:79-154).
+#include <algorithm>
+
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -15,31 +17,44 @@ __device__ __forceinline__ int cloud_of(int32_t B, int64_t g, const int64_t *__r
 // hMax per cloud (computeGridSupport, py/compactHashing/cellTable.py:40-81):
 // gather reads query supports, scatter reference supports, the symmetric
 // modes both.  which = 0: query set, 1: reference set.
+// Each workgroup reduces one contiguous slice of the concatenated supports;
+// a thread keeps a running maximum for its current cloud and flushes it with
+// one atomicMax when the cloud changes, and the wavefront merges lanes that
+// end on the same cloud -- a few thousand atomics for any batch.
+constexpr int kHmaxBlocks = 512;
 __global__ void __launch_bounds__(kBlock) k_hmax(CloudDev *__restrict__ desc, int32_t B,
                                                  const int64_t *__restrict__ start, int64_t n, int which) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
+    const int64_t g0 = (int64_t)blockIdx.x * per, g1 = min(n, g0 + per);
     int c = -1;
-    uint32_t v = 0u;  // ordered encoding: 0 is below every float
-    if (g < n) {
-        c = cloud_of(B, g, start);
-        const CloudDev &cd = desc[c];
-        const int mode = cd.h.mode;
-        // a self-search cloud's query set is its reference set
-        const bool use = which == 0 ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
-                                    : (mode != TCR_MODE_GATHER && !cd.h.self_search);
-        const float *sup = which == 0 ? cd.qs : cd.rs;
-        if (use && sup) v = float_to_ordered(sup[g - start[c]]);
+    uint32_t m = 0u;  // ordered encoding: 0 is below every float
+    bool use = false;
+    const float *sup = nullptr;
+    int64_t c_lo = 0, c_hi = -1;  // [start[c], start[c+1])
+    for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
+        if (g >= c_hi) {
+            if (c >= 0 && m) atomicMax(&desc[c].hmax_ord, m);
+            c = cloud_of(B, g, start);
+            c_lo = start[c];
+            c_hi = start[c + 1];
+            m = 0u;
+            const CloudDev &cd = desc[c];
+            const int mode = cd.h.mode;
+            // a self-search cloud's query set is its reference set
+            use = which == 0 ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
+                             : (mode != TCR_MODE_GATHER && !cd.h.self_search);
+            sup = which == 0 ? cd.qs : cd.rs;
+            if (!sup) use = false;
+        }
+        if (use) m = max(m, float_to_ordered(sup[g - c_lo]));
     }
-    // one atomic per wavefront when the wave lies inside one cloud
     const int c0 = __shfl(c, 0, kWave);
-    const bool same = __all(c == c0 || c < 0);
-    if (same) {
-        uint32_t m = v;
+    if (__all(c == c0)) {
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
         if ((threadIdx.x & (kWave - 1)) == 0 && c0 >= 0 && m) atomicMax(&desc[c0].hmax_ord, m);
-    } else if (c >= 0 && v) {
-        atomicMax(&desc[c].hmax_ord, v);
+    } else if (c >= 0 && m) {
+        atomicMax(&desc[c].hmax_ord, m);
     }
 }
 
@@ -73,45 +88,60 @@ __global__ void k_grid(CloudDev *__restrict__ desc, int32_t B) {
     for (int d = 1; d < dim; ++d) hc = (h[d] > hc || isnan(h[d])) ? h[d] : hc;
     cd.hcell = hc;
     uint32_t err = 0;
+    // wavefront-per-cell eligibility: self-search, stencil radius 1, a
+    // finite positive cell size, and >= 6 cells in every periodic dimension
+    // (so that every candidate in the stencil box has one minimal image)
+    uint32_t fast = (cd.h.self_search && cd.h.search_range == 1 && hc > 0.0f && !isinf(hc)) ? 1u : 0u;
     for (int d = 0; d < 3; ++d) {
         if (d < dim) {
             const float v = ceilf(ext[d] / hc);
             cd.nc[d] = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)v : INT32_MIN;
             if (((cd.h.periodic_mask >> d) & 1u) && cd.nc[d] <= 0 && cd.h.nq > 0 && cd.h.nr > 0) err = 1;
+            if (cd.nc[d] <= 0 || (((cd.h.periodic_mask >> d) & 1u) && cd.nc[d] < 6)) fast = 0;
         } else {
             cd.nc[d] = 1;
         }
     }
     cd.err = err;
+    cd.fast = fast;
 }
 
 // Reference set: second wrap (cppSearch.py:221) on top of radiusSearch's
 // first (neighborhood.py:203), cell coordinates and int32 linear key
 // (cellTable.py:30-31).  Keys are stored sign-flipped so that an unsigned
 // radix sort yields the signed order of torch.argsort.
+// Stray bits for the wavefront-per-cell path: bit 0 -- the second wrap moved
+// the point (its query copy, wrapped once, differs from its sorted copy);
+// bit 1 -- its cell lies outside [0, numCells) in some dimension.
 template <int DIM>
-__device__ __forceinline__ void ref_key_one(const CloudDev &cd, int64_t local, float4 &wp, int32_t &lin) {
+__device__ __forceinline__ void ref_key_one(const CloudDev &cd, int64_t local, float4 &wp, int32_t &lin,
+                                            uint8_t &flags) {
     const float *p = cd.rp + local * DIM;
     float x[3] = {0.f, 0.f, 0.f};
     int32_t c[3];
+    uint8_t f = 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         float v = p[d];
         if ((cd.h.periodic_mask >> d) & 1u) {
             const float lo = cd.h.domain_min[d], ext = cd.h.domain_max[d] - cd.h.domain_min[d];
-            v = wrap_coord(wrap_coord(v, lo, ext), lo, ext);
+            const float w1 = wrap_coord(v, lo, ext);
+            v = wrap_coord(w1, lo, ext);
+            if (__float_as_uint(w1) != __float_as_uint(v)) f |= 1;
         }
         x[d] = v;
         c[d] = cell_coord(v, cd.h.domain_min[d], cd.hcell);
+        if (c[d] < 0 || c[d] >= cd.nc[d]) f |= 2;
     }
     lin = linear_index<DIM>(c, cd.nc);
     wp = make_float4(x[0], x[1], x[2], cd.rs ? cd.rs[local] : 0.0f);
+    flags = f;
 }
 
 __global__ void __launch_bounds__(kBlock) k_ref_keys(const CloudDev *__restrict__ desc, int32_t B,
                                                      const int64_t *__restrict__ r_start, int64_t n,
                                                      float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
-                                                     uint32_t *__restrict__ vals) {
+                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= n) return;
     const int c = cloud_of(B, g, r_start);
@@ -119,14 +149,16 @@ __global__ void __launch_bounds__(kBlock) k_ref_keys(const CloudDev *__restrict_
     const int64_t local = g - r_start[c];
     float4 wp;
     int32_t lin;
+    uint8_t f;
     switch (cd.h.dim) {
-        case 1: ref_key_one<1>(cd, local, wp, lin); break;
-        case 2: ref_key_one<2>(cd, local, wp, lin); break;
-        default: ref_key_one<3>(cd, local, wp, lin); break;
+        case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
+        case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
+        default: ref_key_one<3>(cd, local, wp, lin, f); break;
     }
     rwrap[g] = wp;
     keys[g] = ((uint32_t)lin) ^ 0x80000000u;
     vals[g] = (uint32_t)local;
+    pflags[g] = f;
 }
 
 // sortedPositions = x[sortIndex] (hashTable.py:109), sortedSupports
@@ -138,13 +170,16 @@ __global__ void __launch_bounds__(kBlock) k_gather_sorted(const CloudDev *__rest
                                                           const uint32_t *__restrict__ vals,
                                                           const float4 *__restrict__ rwrap,
                                                           float4 *__restrict__ sorted, int32_t *__restrict__ sidx,
-                                                          int32_t *__restrict__ skey, uint32_t *__restrict__ flags) {
+                                                          int32_t *__restrict__ skey, uint32_t *__restrict__ flags,
+                                                          const uint8_t *__restrict__ pflags,
+                                                          uint8_t *__restrict__ sflags) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const int c = cloud_of(B, s, r_start);
     const uint32_t v = vals[s];
     const int32_t k = (int32_t)(keys[s] ^ 0x80000000u);
     sorted[s] = rwrap[r_start[c] + v];
+    sflags[s] = pflags[r_start[c] + v];
     sidx[s] = (int32_t)v;
     skey[s] = k;
     const bool first = (s == r_start[c]) || ((int32_t)(keys[s - 1] ^ 0x80000000u) != k);
@@ -262,23 +297,24 @@ static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)
 int build_structure(const Batch &b, Work &w, hipStream_t s) {
     const int32_t B = b.B;
     if (b.nq_total > 0)
-        hipLaunchKernelGGL(k_hmax, dim3(blocks_for(b.nq_total)), dim3(kBlock), 0, s, w.desc, B,
-                           (const int64_t *)w.q_start, b.nq_total, 0);
+        hipLaunchKernelGGL(k_hmax, dim3((unsigned)std::min<int64_t>(kHmaxBlocks, blocks_for(b.nq_total))),
+                           dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.q_start, b.nq_total, 0);
     if (b.nr_total > 0)
-        hipLaunchKernelGGL(k_hmax, dim3(blocks_for(b.nr_total)), dim3(kBlock), 0, s, w.desc, B,
-                           (const int64_t *)w.r_start, b.nr_total, 1);
+        hipLaunchKernelGGL(k_hmax, dim3((unsigned)std::min<int64_t>(kHmaxBlocks, blocks_for(b.nr_total))),
+                           dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.r_start, b.nr_total, 1);
     hipLaunchKernelGGL(k_grid, dim3(blocks_for(B, 64)), dim3(64), 0, s, w.desc, B);
     TCR_LAUNCH_CHECK();
     const int64_t n = b.nr_total;
     if (n > 0) {
         hipLaunchKernelGGL(k_ref_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
-                           (const int64_t *)w.r_start, n, w.rwrap, w.keys0, w.vals0);
+                           (const int64_t *)w.r_start, n, w.rwrap, w.keys0, w.vals0, w.pflags);
         TCR_LAUNCH_CHECK();
         int rc = radix_sort_ref(b, w, 4, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gather_sorted, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
                            (const int64_t *)w.r_start, n, (const uint32_t *)w.keys0, (const uint32_t *)w.vals0,
-                           (const float4 *)w.rwrap, w.sorted, w.sort_index, w.skey, w.flags);
+                           (const float4 *)w.rwrap, w.sorted, w.sort_index, w.skey, w.flags,
+                           (const uint8_t *)w.pflags, w.sflags);
         rc = exclusive_scan<uint32_t, uint32_t>(w.flags, w.cellpos, n, false, w.scan_part, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_cell_emit, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,

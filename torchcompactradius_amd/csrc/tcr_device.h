@@ -29,6 +29,7 @@ struct CloudDev {
     float hcell;
     int32_t nc[3];
     uint32_t err;             // nonzero: degenerate grid
+    uint32_t fast;            // nonzero: wavefront-per-cell path eligible
     int64_t cell_off, ncells; // cells of this cloud in the batch cell arrays
     int64_t qcell_off, nqcells; // query groups (non-self) in the batch
 };

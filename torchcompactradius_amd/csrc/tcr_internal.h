@@ -95,7 +95,11 @@ struct Work {
     int32_t *cell_table;  // [Nr*3] reference layout, grouped by bucket
     int32_t *hash_table;  // [HB*2]
     int32_t *counts;      // [Nq]
-    uint32_t *misc;       // small counters
+    uint32_t *misc;       // small counters: [0] cells, [1] strays
+    uint8_t *pflags;      // [Nr] stray bits per reference point (input order)
+    uint8_t *sflags;      // [Nr] stray bits per sorted slot
+    int32_t *cursor;      // [Nq] per-query write cursor across chunks
+    int32_t *stray_list;  // [Nq] global query indices for the per-query path
 };
 
 // One cloud's grid and tables as the traversal reads them.

@@ -135,44 +135,90 @@ __device__ __forceinline__ void query_one(const CloudDev &cd, int64_t g, int64_t
     }
 }
 
+// Clouds whose queries the wavefront-per-cell kernels own (their strays
+// come back through the stray list).
+__device__ __forceinline__ bool group_owned(const CloudDev &cd) {
+    return cd.h.self_search && cd.h.search_range == 1;
+}
+
+struct GenericArgs {
+    const CloudDev *desc;
+    int32_t B;
+    const int64_t *q_start, *r_start;
+    int64_t nq;
+    const int32_t *hash_table, *cell_table;
+    const float4 *sorted;
+    const int32_t *sort_index;
+    int32_t *counts;
+    const int64_t *offsets;
+    int64_t *row, *col;
+    const int32_t *list;        // stray list (list mode)
+    const uint32_t *list_count;
+};
+
 template <bool FILL>
-__global__ void __launch_bounds__(kQBlock)
-k_query_generic(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restrict__ q_start,
-                const int64_t *__restrict__ r_start, int64_t nq, const int32_t *__restrict__ hash_table,
-                const int32_t *__restrict__ cell_table, const float4 *__restrict__ sorted,
-                const int32_t *__restrict__ sort_index, int32_t *__restrict__ counts,
-                const int64_t *__restrict__ offsets, int64_t *__restrict__ row, int64_t *__restrict__ col) {
-    const int64_t g = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
-    if (g >= nq) return;
-    const int c = find_segment(B, g, [&](int k) { return q_start[k]; });
-    const CloudDev &cd = desc[c];
-    const int64_t local = g - q_start[c];
+__device__ __forceinline__ void generic_query(const GenericArgs &a, int64_t g) {
+    const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
+    const CloudDev &cd = a.desc[c];
+    const int64_t local = g - a.q_start[c];
     switch (cd.h.dim) {
-        case 1: query_one<1, FILL>(cd, g, local, hash_table, cell_table, sorted, r_start + c, sort_index, counts, offsets, row, col); break;
-        case 2: query_one<2, FILL>(cd, g, local, hash_table, cell_table, sorted, r_start + c, sort_index, counts, offsets, row, col); break;
-        default: query_one<3, FILL>(cd, g, local, hash_table, cell_table, sorted, r_start + c, sort_index, counts, offsets, row, col); break;
+        case 1: query_one<1, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
+        case 2: query_one<2, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
+        default: query_one<3, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
     }
 }
 
-int query_count_generic(const Batch &b, Work &w, hipStream_t s) {
-    if (b.nq_total == 0) return TCR_OK;
-    hipLaunchKernelGGL(k_query_generic<false>, dim3((unsigned)((b.nq_total + kQBlock - 1) / kQBlock)),
-                       dim3(kQBlock), 0, s, (const CloudDev *)w.desc, b.B, (const int64_t *)w.q_start,
-                       (const int64_t *)w.r_start, b.nq_total, (const int32_t *)w.hash_table,
-                       (const int32_t *)w.cell_table, (const float4 *)w.sorted, (const int32_t *)w.sort_index,
-                       w.counts, (const int64_t *)nullptr, (int64_t *)nullptr, (int64_t *)nullptr);
-    TCR_LAUNCH_CHECK();
-    return TCR_OK;
+// Every query of the clouds the wavefront-per-cell kernels do not own.
+template <bool FILL>
+__global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
+    const int64_t g = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
+    if (g >= a.nq) return;
+    const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
+    if (group_owned(a.desc[c])) return;
+    generic_query<FILL>(a, g);
 }
 
-int query_fill_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
-                       hipStream_t s) {
-    if (b.nq_total == 0) return TCR_OK;
-    hipLaunchKernelGGL(k_query_generic<true>, dim3((unsigned)((b.nq_total + kQBlock - 1) / kQBlock)),
-                       dim3(kQBlock), 0, s, (const CloudDev *)w.desc, b.B, (const int64_t *)w.q_start,
-                       (const int64_t *)w.r_start, b.nq_total, (const int32_t *)w.hash_table,
-                       (const int32_t *)w.cell_table, (const float4 *)w.sorted, (const int32_t *)w.sort_index,
-                       w.counts, offsets, row, col);
+// The stray list left by the wavefront-per-cell count pass.
+template <bool FILL>
+__global__ void __launch_bounds__(kQBlock) k_query_list(GenericArgs a) {
+    const int64_t n = *a.list_count;
+    for (int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kQBlock)
+        generic_query<FILL>(a, a.list[i]);
+}
+
+static GenericArgs generic_args(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col) {
+    GenericArgs a;
+    a.desc = w.desc;
+    a.B = b.B;
+    a.q_start = w.q_start;
+    a.r_start = w.r_start;
+    a.nq = b.nq_total;
+    a.hash_table = w.hash_table;
+    a.cell_table = w.cell_table;
+    a.sorted = w.sorted;
+    a.sort_index = w.sort_index;
+    a.counts = w.counts;
+    a.offsets = offsets;
+    a.row = row;
+    a.col = col;
+    a.list = w.stray_list;
+    a.list_count = w.misc + 1;
+    return a;
+}
+
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
+                   bool list, hipStream_t s) {
+    const GenericArgs a = generic_args(b, w, offsets, row, col);
+    if (list) {
+        const dim3 grid(1024), block(kQBlock);
+        if (fill) hipLaunchKernelGGL(k_query_list<true>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(k_query_list<false>, grid, block, 0, s, a);
+    } else {
+        if (b.nq_total == 0) return TCR_OK;
+        const dim3 grid((unsigned)((b.nq_total + kQBlock - 1) / kQBlock)), block(kQBlock);
+        if (fill) hipLaunchKernelGGL(k_query_generic<true>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(k_query_generic<false>, grid, block, 0, s, a);
+    }
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
