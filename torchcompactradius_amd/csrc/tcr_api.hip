@@ -103,6 +103,11 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
     w.sflags = cv.take<uint8_t>(nr);
     w.cursor = cv.take<int32_t>(nq);
     w.stray_list = cv.take<int32_t>(nq);
+    w.masks = cv.take<uint64_t>(7 * (size_t)nr);
+    w.ranges = cv.take<int2>(27 * (size_t)nr);
+    w.recs = cv.take<float4>(2 * (size_t)nr);
+    w.soff = cv.take<int64_t>(nr);
+    w.big_list = cv.take<uint32_t>(nr);
     return cv.off + 256;
 }
 

@@ -1,30 +1,53 @@
 // Wavefront-per-cell count and fill passes for self-search clouds (the query
 // set is the reference set), searchRange 1.
 //
-// A wavefront owns one occupied cell ("group") at a time.  Its lanes look up
-// the 3^d stencil cells once through the compact hash table (same lookup
-// semantics as queryHashMap, cpp/neighborhood.h:35-58), lay the candidates
-// out in the reference's order -- stencil offset order, then sorted position
-// -- across lanes and rounds, and then loop over the group's query points
-// with the query held uniform.  Per (query, round) the 64 candidate tests
-// give one hit mask; the count pass sums its popcounts, the fill pass writes
-// the hits with a lane prefix (mbcnt) so that a row comes out in exactly the
-// reference order (cpp/neighborhoodDynamic.h:76-113).
+// 1. k_stencil (one thread per cell and stencil offset) resolves the 3^d
+//    neighbour cells of every occupied cell through the compact hash table,
+//    with the lookup semantics of queryHashMap (cpp/neighborhood.h:35-58):
+//    periodic coordinates through pymod, non-periodic ones unclamped, the
+//    bucket scan matching only the linear index.  It also packs a 32-byte
+//    record per cell (points, cloud, state, centre).
+// 2. k_count_small (one wavefront per cell with <= kSmallChunk candidates):
+//    the lanes lay the candidates out in the reference's order -- stencil
+//    offset order (dim 0 slowest), then sorted position -- over rounds of 64
+//    and loop over the cell's query points with the query held uniform.  Per
+//    (query, round) the 64 tests give one hit mask; the pass stores the
+//    masks and the per-query counts.
+// 3. k_fill_small replays the masks: each query's hits are packed in LDS at
+//    the lane prefix count of the mask, so a row comes out in exactly the
+//    reference order (cpp/neighborhoodDynamic.h:76-113), and leave in
+//    coalesced stores -- no second distance evaluation.
+// 4. Cells with more candidates go through k_group_big (count and fill, in
+//    chunks, recomputing the tests in the fill).
+// Both small-cell kernels walk their cells software-pipelined: the next
+// cell's record and stencil ranges are in flight while the current cell
+// computes.
 //
-// Predicate: a filtered-exact test.  An fp32 estimate of the squared
-// distance with a rigorous error margin decides "in" or "out"; only pairs in
-// the margin band (and candidates outside the stencil box, e.g. aliased
-// out-of-domain points) take the exact fp64 modDistance + sqrtf + '<' path
-// of the reference (cpp/common.h:106-129).  Queries whose once-wrapped
-// position differs from their twice-wrapped sorted copy, or whose cell lies
-// outside [0, numCells), are "strays": they go to the per-query kernel.
+// Predicate: a filtered-exact test.  Coordinates are centred on the cell and
+// candidates moved to the periodic image nearest to it; the squared distance
+// is estimated as |c|^2 - 2 q.c + |q|^2 with a rigorous error margin that
+// covers the centring and image shift, the expansion's cancellation, the
+// reference's own fp32 difference and fp64/fp32 accumulation, and the sqrtf
+// rounding.  Pairs inside the margin band, and candidates outside the 3^d
+// stencil box (e.g. aliased out-of-domain points), take the exact fp64
+// modDistance + sqrtf + '<' path of the reference (cpp/common.h:106-129).
+// Queries whose once-wrapped position differs from their twice-wrapped
+// sorted copy, or whose cell lies outside [0, numCells), are "strays" and go
+// to the per-query kernel, as do all points of a cell whose first point is
+// outside the grid.
 #include "tcr_internal.h"
 
 namespace tcr {
 
 constexpr int kGroupThreads = 256;           // 4 wavefronts per workgroup
-constexpr int kRounds = 8;                   // candidate rounds per chunk
-constexpr int kChunk = kRounds * kWave;      // 512 candidates per chunk
+constexpr int kSmallRounds = 7;              // candidate rounds of a small cell
+constexpr int kSmallChunk = kSmallRounds * kWave;  // 448
+constexpr int kBigRounds = 8;
+constexpr int kBigChunk = kBigRounds * kWave;      // 512 candidates per chunk
+constexpr int kStencil = 27;                 // stride of the per-cell stencil table
+constexpr float kU = 0x1p-24f;               // fp32 unit roundoff
+
+enum : uint32_t { kCellSkip = 0, kCellFast = 1, kCellStray = 2 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
@@ -38,41 +61,63 @@ __device__ __forceinline__ T uniform(T v) {
 __device__ __forceinline__ float uniform_f(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
+// Lane m's value, m uniform (v_readlane).
+__device__ __forceinline__ int32_t bcast(int32_t v, uint32_t m) { return __builtin_amdgcn_readlane(v, m); }
+__device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t m) {
+    return (uint32_t)__builtin_amdgcn_readlane((int32_t)v, m);
+}
+__device__ __forceinline__ float bcast(float v, uint32_t m) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), m));
+}
+__device__ __forceinline__ uint64_t bcast(uint64_t v, uint32_t m) {
+    const uint32_t lo = bcast((uint32_t)v, m), hi = bcast((uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t bcast(int64_t v, uint32_t m) { return (int64_t)bcast((uint64_t)v, m); }
 
-// Thresholds on the squared distance for "dist < h": in if s < lo, out if
-// s >= hi, exact test in between.  rel covers the fp32 estimate's and the
-// reference's rounding (<= 10 ulp relative) plus sqrtf rounding; mabs covers
-// the absolute error of shifted periodic coordinates (delta per dim).
+// Decision thresholds on the estimated squared distance for "dist < h":
+// in if est < lo, out if est >= hi, exact test in between.
 struct Thr {
     float lo, hi;
 };
-__device__ __forceinline__ Thr thresholds(float h, float delta, int dim) {
+__device__ __forceinline__ Thr thresholds(float h, float geo, float cell2, int dim) {
     Thr t;
     if (!(h > 0.0f)) {          // h <= 0 or NaN: no distance is below it
-        t.lo = -1.0f;
-        t.hi = -1.0f;
+        t.lo = -INFINITY;
+        t.hi = -INFINITY;
     } else if (isinf(h)) {      // every finite distance is below +inf
         t.lo = INFINITY;
         t.hi = INFINITY;
     } else {
         const float h2 = h * h;
-        if (!(h2 > 1e-30f)) {   // tiny supports: decide every pair exactly
-            t.lo = -1.0f;
+        if (!(h2 > 0x1p-100f)) {  // tiny supports: decide every pair exactly
+            t.lo = -INFINITY;
             t.hi = INFINITY;
         } else {
-            const float mabs = (float)dim * (2.0f * h * delta + delta * delta) * 1.01f;
-            t.lo = h2 * (1.0f - 0x1p-18f) - mabs;
-            t.hi = h2 * (1.0f + 0x1p-18f) + mabs;
+            // twice the bound: 80u cell^2 (centring + expansion), 4u h^2
+            // (both accumulations, sqrtf, h*h), 2 dim h u geo (fp32
+            // differences and image shifts of the reference and the estimate)
+            const float m = 2.0f * kU * (80.0f * cell2 + 4.0f * h2 + 2.0f * (float)dim * h * geo);
+            t.lo = h2 - m;
+            t.hi = h2 + m;
         }
     }
     return t;
 }
 
+// Per-cell record written by k_stencil.
+struct __align__(16) CellRec {
+    uint32_t gb, ge;    // sorted slots [gb, ge) of the cell's points
+    uint32_t cloud;
+    uint32_t state;     // kCellSkip / kCellFast / kCellStray
+    float4 center;
+};
+
 struct GroupArgs {
     const CloudDev *desc;
     const int64_t *r_start;
     const int64_t *q_start;
-    const uint32_t *misc;        // [0] = occupied cells in the batch
+    const uint32_t *misc;        // [0] occupied cells, [1] strays, [2] big cells
     const uint32_t *cell_begin;  // [cells+1] first global sorted slot
     const uint32_t *cell_cloud;
     const int32_t *hash_table;
@@ -80,200 +125,498 @@ struct GroupArgs {
     const float4 *sorted;
     const int32_t *sort_index;   // local original index per sorted slot
     const uint8_t *sflags;       // per sorted slot: nonzero = stray query
+    int2 *ranges;                // [cells * 27] (first global slot, count)
+    CellRec *recs;               // [cells]
+    uint64_t *masks;             // [nr * kSmallRounds]
     int32_t *counts;             // [nq] per global query index
     const int64_t *offsets;      // [nq+1] (fill)
+    int64_t *soff;               // [nr] row offset of each sorted slot's query (fill)
     int64_t *row, *col;          // fill outputs (row may be null)
     int32_t *cursor;             // [nq] per-query write cursor across chunks (fill)
     int32_t *stray_list;         // [nq]
     uint32_t *stray_count;
-    int add_strays;              // count pass appends strays
+    uint32_t *big_list;          // [cells]
+    uint32_t *big_count;
 };
 
-template <int DIM, int MODE, bool FILL>
-__device__ void group_one(const GroupArgs &a, uint32_t cid) {
-    const int lane = __lane_id();
-    const uint32_t c = uniform(a.cell_cloud[cid]);
-    const CloudDev &cd = a.desc[c];
-    const int64_t rs = uniform(a.r_start[c]);
-    const int64_t qs = uniform(a.q_start[c]);
-    const uint32_t gb = uniform(a.cell_begin[cid]), ge = uniform(a.cell_begin[cid + 1]);
+// ------------------------------------------------------------ stencil ----
 
-    float mn[3], mx[3], ext[3];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        mn[d] = cd.h.domain_min[d];
-        mx[d] = cd.h.domain_max[d];
-        ext[d] = mx[d] - mn[d];
-    }
-    const uint32_t pmask = cd.h.periodic_mask;
-    const float hcell = cd.hcell;
-    const bool cloud_fast = cd.fast != 0;
+__device__ __forceinline__ bool group_owned(const CloudDev &cd) {
+    return cd.h.self_search && cd.h.search_range == 1;
+}
 
-    // group cell from its first point; a class whose first point is outside
-    // the grid (aliased linear index) is handled entirely per query
-    const float4 p0 = a.sorted[gb];
-    const float p0v[3] = {p0.x, p0.y, p0.z};
-    int32_t q[3];
+template <int DIM>
+__device__ __forceinline__ bool cell_coords(const CloudDev &cd, float4 p0, int32_t *q) {
+    const float v[3] = {p0.x, p0.y, p0.z};
     bool inrange = true;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        q[d] = cell_coord(p0v[d], mn[d], hcell);
+        q[d] = cell_coord(v[d], cd.h.domain_min[d], cd.hcell);
         inrange = inrange && q[d] >= 0 && q[d] < cd.nc[d];
     }
-    if (!cloud_fast || !inrange) {
-        if (a.add_strays)
-            for (uint32_t s = gb + lane; s < ge; s += kWave) {
-                const uint32_t k = atomicAdd(a.stray_count, 1u);
-                a.stray_list[k] = (int32_t)(qs + a.sort_index[s]);
-            }
-        return;
-    }
+    return inrange;
+}
 
-    // ---- stencil lookup: lane o < 3^DIM resolves offset o (dim 0 slowest)
-    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
-    int32_t cbeg = 0, ccnt = 0;
-    if (lane < kOff) {
-        int32_t cc[3];
-        int32_t rem = lane;
+template <int DIM>
+__device__ __forceinline__ int2 stencil_one(const GroupArgs &a, const CloudDev &cd, const int32_t *q, int o,
+                                            int64_t rs) {
+    int32_t cc[3];
+    int32_t rem = o;
 #pragma unroll
-        for (int d = DIM - 1; d >= 0; --d) {
-            const int32_t off = (rem % 3) - 1;
-            rem /= 3;
-            cc[d] = ((pmask >> d) & 1u) ? pymod(q[d] + off, cd.nc[d]) : q[d] + off;
-        }
-        const int32_t lin = linear_index<DIM>(cc, cd.nc);
-        const uint32_t hb = hash_index<DIM>(cc, cd.h.hash_map_length);
-        const int32_t *ht = a.hash_table + 2 * cd.hb_off;
-        const int32_t *ct = a.cell_table + 3 * cd.cell_off;
-        const int32_t b0 = ht[2 * (int64_t)hb], bl = ht[2 * (int64_t)hb + 1];
-        if (b0 != -1)
-            for (int32_t i = b0; i < b0 + bl; ++i)
-                if (ct[3 * (int64_t)i] == lin) {
-                    cbeg = ct[3 * (int64_t)i + 1];
-                    ccnt = ct[3 * (int64_t)i + 2];
-                    break;
-                }
+    for (int d = DIM - 1; d >= 0; --d) {
+        const int32_t off = (rem % 3) - 1;
+        rem /= 3;
+        cc[d] = ((cd.h.periodic_mask >> d) & 1u) ? pymod(q[d] + off, cd.nc[d]) : q[d] + off;
     }
-    // inclusive scan of the stencil cell sizes over lanes
-    int32_t incl = ccnt;
+    const int32_t lin = linear_index<DIM>(cc, cd.nc);
+    const uint32_t hb = hash_index<DIM>(cc, cd.h.hash_map_length);
+    const int32_t *ht = a.hash_table + 2 * cd.hb_off;
+    const int32_t *ct = a.cell_table + 3 * cd.cell_off;
+    const int32_t b0 = ht[2 * (int64_t)hb], bl = ht[2 * (int64_t)hb + 1];
+    if (b0 != -1)
+        for (int32_t i = b0; i < b0 + bl; ++i)
+            if (ct[3 * (int64_t)i] == lin)
+                return make_int2((int32_t)(rs + ct[3 * (int64_t)i + 1]), ct[3 * (int64_t)i + 2]);
+    return make_int2(0, 0);
+}
+
+__global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
+    const uint32_t ncells = a.misc[0];
+    const int64_t n = (int64_t)ncells * kStencil;
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
+        const uint32_t cid = (uint32_t)(g / kStencil);
+        const int o = (int)(g - (int64_t)cid * kStencil);
+        const uint32_t c = a.cell_cloud[cid];
+        const CloudDev &cd = a.desc[c];
+        const int dim = cd.h.dim;
+        const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
+        const uint32_t gb = a.cell_begin[cid];
+        if (!group_owned(cd)) {
+            if (o == 0) a.recs[cid] = CellRec{gb, a.cell_begin[cid + 1], c, kCellSkip, make_float4(0, 0, 0, 0)};
+            continue;
+        }
+        const float4 p0 = a.sorted[gb];
+        int32_t q[3];
+        const bool inrange = dim == 1 ? cell_coords<1>(cd, p0, q) : dim == 2 ? cell_coords<2>(cd, p0, q)
+                                                                             : cell_coords<3>(cd, p0, q);
+        const bool fast = inrange && cd.fast;
+        if (o == 0) {
+            float cen[3] = {0.f, 0.f, 0.f};
+            for (int d = 0; d < dim; ++d) cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
+            a.recs[cid] = CellRec{gb, a.cell_begin[cid + 1], c, fast ? kCellFast : kCellStray,
+                                  make_float4(cen[0], cen[1], cen[2], 0.0f)};
+        }
+        int2 r = make_int2(0, 0);
+        if (fast && o < noff) {
+            const int64_t rs = a.r_start[c];
+            r = dim == 1 ? stencil_one<1>(a, cd, q, o, rs) : dim == 2 ? stencil_one<2>(a, cd, q, o, rs)
+                                                                      : stencil_one<3>(a, cd, q, o, rs);
+        }
+        a.ranges[g] = r;
+    }
+}
+
+// ------------------------------------------------------- per-cell setup ---
+
+// One cell as a wavefront sees it: the record, and lane o holding stencil
+// cell o's first global slot and exclusive candidate prefix.
+struct CellView {
+    uint32_t c, gb, ge, state;
+    float4 center;
+    int64_t qs;          // first global query index of the cloud
+    int32_t rbeg, pre;   // per lane (stencil cell `lane`)
+    int32_t total;       // candidates
+};
+
+struct CellLoad {
+    CellRec rec;
+    int2 rng;
+};
+
+__device__ __forceinline__ CellLoad load_cell(const GroupArgs &a, uint32_t cid) {
+    CellLoad l;
+    l.rec = a.recs[cid];
+    l.rng = __lane_id() < kStencil ? a.ranges[(int64_t)cid * kStencil + __lane_id()] : make_int2(0, 0);
+    return l;
+}
+
+__device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad &l, int noff) {
+    const int lane = __lane_id();
+    CellView v;
+    v.gb = uniform(l.rec.gb);
+    v.ge = uniform(l.rec.ge);
+    v.c = uniform(l.rec.cloud);
+    v.state = uniform(l.rec.state);
+    v.center = make_float4(uniform_f(l.rec.center.x), uniform_f(l.rec.center.y), uniform_f(l.rec.center.z), 0.f);
+    v.qs = uniform(a.q_start[v.c]);
+    const int32_t cnt = lane < noff ? l.rng.y : 0;
+    int32_t incl = cnt;
 #pragma unroll
     for (int o = 1; o < 32; o <<= 1) {
         const int32_t n = __shfl_up(incl, o, kWave);
         if (lane >= o) incl += n;
     }
-    const int32_t pre = incl - ccnt;                 // exclusive prefix of cell `lane`
-    const int32_t total = uniform(__shfl(incl, kOff - 1, kWave));
+    v.rbeg = l.rng.x;
+    v.pre = incl - cnt;
+    v.total = uniform(__shfl(incl, noff - 1, kWave));
+    return v;
+}
 
-    // geometry of the group cell for image shifts and the stencil-box check
-    float center[3], delta = 0.0f;
+// Global sorted slot of candidate k (k < total), by binary search over the
+// stencil prefix (used where no LDS slot table is built).
+__device__ __forceinline__ int32_t candidate_slot(const CellView &v, int32_t k, int noff) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1) {
+        const int mid = lo + step;
+        const int32_t pm = __shfl(v.pre, mid < noff ? mid : noff - 1, kWave);
+        if (mid < noff && pm <= k) lo = mid;
+    }
+    return __shfl(v.rbeg, lo, kWave) + (k - __shfl(v.pre, lo, kWave));
+}
+
+// Per-wavefront LDS table of a small cell: slot_off[k] = first global slot of
+// the stencil cell holding candidate k minus its candidate prefix, so that
+// candidate k lives at global slot k + slot_off[k].  Lane o writes the
+// entries of stencil cell o.
+__device__ __forceinline__ void expand_slots(const CellView &v, int32_t *slot_off, int noff) {
+    const int lane = __lane_id();
+    const int32_t next = __shfl_down(v.pre, 1, kWave);
+    const int32_t cnt = lane < noff ? (lane == noff - 1 ? v.total - v.pre : next - v.pre) : 0;
+    const int32_t d = v.rbeg - v.pre;
+    for (int32_t t = 0; __any(t < cnt); ++t)
+        if (t < cnt) slot_off[v.pre + t] = d;
+    __asm__ volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void append_strays(const GroupArgs &a, const CellView &v) {
+    for (uint32_t s = v.gb + __lane_id(); s < v.ge; s += kWave) {
+        const uint32_t k = atomicAdd(a.stray_count, 1u);
+        a.stray_list[k] = (int32_t)(v.qs + a.sort_index[s]);
+    }
+}
+
+// Geometry of the filter for one cell.
+template <int DIM>
+struct Geo {
+    float ext[3], center[3];
+    float box, geo, cell2;
+    uint32_t pmask;
+    __device__ __forceinline__ Geo(const CloudDev &cd, float4 cen) {
+        pmask = cd.h.periodic_mask;
+        const float hcell = cd.hcell;
+        const float cv[3] = {cen.x, cen.y, cen.z};
+        float coord_max = 0.0f, ext_max = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const float mn = cd.h.domain_min[d], mx = cd.h.domain_max[d];
+            ext[d] = mx - mn;
+            center[d] = cv[d];
+            coord_max = fmaxf(coord_max, fmaxf(fabsf(mn), fabsf(mx)));
+            if ((pmask >> d) & 1u) ext_max = fmaxf(ext_max, ext[d]);
+        }
+        // candidates of the stencil lie within 1.5 cells of the centre,
+        // queries of the cell within 0.5 cells
+        box = 1.51f * hcell;
+        geo = 3.0f * (coord_max + 2.0f * box) + ext_max + 3.0f * hcell;
+        cell2 = hcell * hcell;
+    }
+    // Candidate at its image nearest to the cell, centred, with |c|^2 in p:
+    // +inf for a lane without a candidate, NaN for a candidate outside the
+    // stencil box (then the pair is decided exactly).
+    __device__ __forceinline__ void place(float4 v, bool valid, float *xc, float &p) const {
+        const float xv[3] = {v.x, v.y, v.z};
+        bool well = true;
+        p = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            float x = xv[d];
+            if ((pmask >> d) & 1u) {
+                const float e0 = x - center[d];
+                if (e0 > 0.5f * ext[d]) x = x - ext[d];
+                else if (e0 < -0.5f * ext[d]) x = x + ext[d];
+            }
+            const float c = x - center[d];
+            well = well && fabsf(c) <= box;
+            xc[d] = c;
+            p = d == 0 ? c * c : fmaf(c, c, p);
+        }
+        if (!valid) p = INFINITY;
+        else if (!well) p = __int_as_float(0x7fc00000);
+    }
+};
+
+// The reference's exact test for one pair; the domain is re-read from the
+// descriptor so that nothing of it stays live in the hot loops (rare path).
+template <int DIM, int MODE>
+__device__ __attribute__((noinline)) bool exact_pair(const GroupArgs &a, const CloudDev *cd, uint32_t qs_slot,
+                                                     int32_t j) {
+    const float4 xq = a.sorted[qs_slot];
+    const float xi[3] = {xq.x, xq.y, xq.z};
+    const float4 v = a.sorted[j];
+    const float xj[3] = {v.x, v.y, v.z};
+    float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        center[d] = mn[d] + ((float)q[d] + 0.5f) * hcell;
-        if ((pmask >> d) & 1u) {
-            const float m = fmaxf(fabsf(mn[d]), fabsf(mx[d]));
-            delta = fmaxf(delta, 0x1p-22f * (m + ext[d]));
+        mn[d] = cd->h.domain_min[d];
+        mx[d] = cd->h.domain_max[d];
+    }
+    const float dist = exact_distance<DIM>(xi, xj, mn, mx, cd->h.periodic_mask);
+    return support_pred(dist, xq.w, v.w, MODE);
+}
+
+// One round of tests of query q against the 64 candidates in lanes:
+// returns the hit mask.  In: est < lo; exact band: not in and not
+// (est >= hi) -- NaN candidates land in the band, +inf lanes never hit.
+template <int DIM, int MODE, class SlotOf>
+__device__ __forceinline__ uint64_t round_hits(const GroupArgs &a, const CloudDev &cd, const Geo<DIM> &g,
+                                               const float *cx, float cp, float ch, const Thr &tjr,
+                                               const float *xi2, float pi, float hi, const Thr &tq,
+                                               uint32_t q_slot, SlotOf slot_of) {
+    float t = cp;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) t = fmaf(cx[d], xi2[d], t);
+    Thr th;
+    if (MODE == TCR_MODE_GATHER) {
+        th = tq;
+    } else {
+        if (MODE == TCR_MODE_SCATTER) th = tjr;
+        else if (MODE == TCR_MODE_SYMMETRIC) th = thresholds((hi + ch) / 2.f, g.geo, g.cell2, DIM);
+        else th = thresholds(max_ref(hi, ch), g.geo, g.cell2, DIM);
+        t += pi;
+    }
+    uint64_t hitm = __ballot(t < th.lo);
+    const uint64_t band = __ballot(!(t >= th.hi)) & ~hitm;
+    if (band) {
+        // the candidate's slot is only needed here; evaluated with every
+        // lane active (the lookup may read other lanes)
+        const int32_t jslot = slot_of();
+        bool ok = false;
+        if ((band >> __lane_id()) & 1ull) ok = exact_pair<DIM, MODE>(a, &cd, q_slot, jslot);
+        hitm |= __ballot(ok) & band;
+    }
+    return hitm;
+}
+
+// ------------------------------------------------------------ small cells --
+
+template <int DIM, int MODE>
+__device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid, int32_t *slot_off) {
+    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    const int lane = __lane_id();
+    const CloudDev &cd = a.desc[v.c];
+    // first member batch: independent of the candidates, issue it first
+    const uint32_t nm0 = min((uint32_t)kWave, v.ge - v.gb);
+    const uint32_t ms0 = (uint32_t)lane < nm0 ? v.gb + lane : v.gb;
+    uint32_t mfl = (uint32_t)lane < nm0 ? a.sflags[ms0] : 0u;
+    int32_t mloc = a.sort_index[ms0];
+    float4 mq = a.sorted[ms0];
+
+    const Geo<DIM> g(cd, v.center);
+    const int nrounds = (v.total + kWave - 1) / kWave;
+    expand_slots(v, slot_off, kOff);
+
+    float cx[kSmallRounds][3], cp[kSmallRounds], chh[kSmallRounds];
+    Thr tj[kSmallRounds];
+#pragma unroll
+    for (int r = 0; r < kSmallRounds; ++r) {
+        const int32_t k = r * kWave + lane;
+        const bool valid = k < v.total;
+        const int32_t j = valid ? k + slot_off[k] : (int32_t)v.gb;
+        const float4 cv = a.sorted[j];
+        g.place(cv, valid, cx[r], cp[r]);
+        chh[r] = cv.w;
+        if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
+    }
+
+    for (uint32_t mb = v.gb; mb < v.ge; mb += kWave) {
+        const uint32_t nm = min((uint32_t)kWave, v.ge - mb);
+        const bool mvalid = (uint32_t)lane < nm;
+        const uint32_t ms = mvalid ? mb + lane : v.gb;
+        if (mb != v.gb) {
+            mfl = mvalid ? a.sflags[ms] : 0u;
+            mloc = a.sort_index[ms];
+            mq = a.sorted[ms];
+        }
+        const int64_t mqg = v.qs + mloc;
+        if (mvalid && mfl) {
+            const uint32_t k = atomicAdd(a.stray_count, 1u);
+            a.stray_list[k] = (int32_t)mqg;
+        }
+        const uint64_t todo = __ballot(mvalid && !mfl);
+        if (!todo) continue;
+        const float mqv[3] = {mq.x, mq.y, mq.z};
+        float mxc[3] = {0.f, 0.f, 0.f}, mpi = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const float c = mqv[d] - g.center[d];
+            mxc[d] = -2.0f * c;
+            mpi = d == 0 ? c * c : fmaf(c, c, mpi);
+        }
+        Thr mt{0.f, 0.f};
+        if (MODE == TCR_MODE_GATHER) {
+            mt = thresholds(mq.w, g.geo, g.cell2, DIM);
+            mt.lo -= mpi;
+            mt.hi -= mpi;
+        }
+        uint64_t mk[kSmallRounds];
+#pragma unroll
+        for (int r = 0; r < kSmallRounds; ++r) mk[r] = 0ull;
+        int32_t mhits = 0;
+
+        for (uint32_t m = 0; m < nm; ++m) {
+            if (!((todo >> m) & 1ull)) continue;
+            const bool mine = (uint32_t)lane == m;
+            float xi2[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xi2[d] = bcast(mxc[d], m);
+            float pi = 0.0f, hi = 0.0f;
+            Thr tq{0.f, 0.f};
+            if (MODE == TCR_MODE_GATHER) {
+                tq = Thr{bcast(mt.lo, m), bcast(mt.hi, m)};
+            } else {
+                pi = bcast(mpi, m);
+                hi = bcast(mq.w, m);
+            }
+            int32_t hits = 0;
+#pragma unroll
+            for (int r = 0; r < kSmallRounds; ++r) {
+                if (r >= nrounds) break;
+                const uint64_t hitm = round_hits<DIM, MODE>(
+                    a, cd, g, cx[r], cp[r], chh[r], tj[r], xi2, pi, hi, tq, mb + m, [&]() {
+                        const int32_t k = r * kWave + lane;
+                        return k < v.total ? k + slot_off[k] : (int32_t)v.gb;
+                    });
+                if (mine) mk[r] = hitm;
+                hits += __popcll(hitm);
+            }
+            if (mine) mhits = hits;
+        }
+        if (mvalid && !mfl) {
+            a.counts[mqg] = mhits;
+#pragma unroll
+            for (int r = 0; r < kSmallRounds; ++r)
+                if (r < nrounds) a.masks[(int64_t)ms * kSmallRounds + r] = mk[r];
         }
     }
-    const float box = 1.51f * hcell;  // stencil box half-width around the centre
+}
 
-    for (int32_t cbase = 0; cbase < total; cbase += kChunk) {
-        // ---- stage this chunk's candidates in registers
-        float cx[kRounds][3];
-        float chh[kRounds];
-        int32_t cj[kRounds];
-        uint64_t vmask[kRounds], exmask[kRounds];
+// Replays the stored masks.  Each query's row is assembled in LDS (rows of
+// a small cell hold at most kSmallChunk entries) and leaves in coalesced
+// stores.  Independent of mode and dimension except for the stencil size.
+__device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int32_t *slot_off, int32_t *rowbuf) {
+    const int lane = __lane_id();
+    const int nrounds = (v.total + kWave - 1) / kWave;
+    // first member batch: independent of the candidates, issue it first
+    const uint32_t nm0 = min((uint32_t)kWave, v.ge - v.gb);
+    const uint32_t ms0 = (uint32_t)lane < nm0 ? v.gb + lane : v.gb;
+    bool live = (uint32_t)lane < nm0 && !a.sflags[ms0];
+    uint64_t mk[kSmallRounds];
 #pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const int32_t k = cbase + r * kWave + lane;
-            const bool valid = k < total;
-            // stencil cell holding candidate k: last o with pre[o] <= k
-            int lo = 0;
+    for (int r = 0; r < kSmallRounds; ++r)
+        mk[r] = (r < nrounds && live) ? a.masks[(int64_t)ms0 * kSmallRounds + r] : 0ull;
+    int64_t mbase = live ? a.soff[ms0] : 0;
+    int32_t mloc = a.sort_index[ms0];
+
+    expand_slots(v, slot_off, noff);
+    int32_t corig[kSmallRounds];
 #pragma unroll
-            for (int step = 16; step > 0; step >>= 1) {
-                const int mid = lo + step;
-                const int32_t pm = __shfl(pre, mid < kOff ? mid : kOff - 1, kWave);
-                if (mid < kOff && pm <= k) lo = mid;
+    for (int r = 0; r < kSmallRounds; ++r) {
+        const int32_t k = r * kWave + lane;
+        corig[r] = k < v.total ? a.sort_index[k + slot_off[k]] : 0;
+    }
+    for (uint32_t mb = v.gb; mb < v.ge; mb += kWave) {
+        const uint32_t nm = min((uint32_t)kWave, v.ge - mb);
+        const bool mvalid = (uint32_t)lane < nm;
+        const uint32_t ms = mvalid ? mb + lane : v.gb;
+        if (mb != v.gb) {
+            live = mvalid && !a.sflags[ms];
+#pragma unroll
+            for (int r = 0; r < kSmallRounds; ++r)
+                mk[r] = (r < nrounds && live) ? a.masks[(int64_t)ms * kSmallRounds + r] : 0ull;
+            mbase = live ? a.soff[ms] : 0;
+            mloc = a.sort_index[ms];
+        }
+        const uint64_t todo = __ballot(live);
+        if (!todo) continue;
+        for (uint32_t m = 0; m < nm; ++m) {
+            if (!((todo >> m) & 1ull)) continue;
+            int32_t len = 0;
+#pragma unroll
+            for (int r = 0; r < kSmallRounds; ++r) {
+                if (r >= nrounds) break;
+                const uint64_t hitm = bcast(mk[r], m);
+                if ((hitm >> lane) & 1ull) rowbuf[len + __popcll(hitm & lanemask_lt())] = corig[r];
+                len += __popcll(hitm);
             }
-            const int32_t ob = __shfl(cbeg, lo, kWave), op = __shfl(pre, lo, kWave);
-            const int32_t j = valid ? (int32_t)rs + ob + (k - op) : (int32_t)rs;
-            cj[r] = j;
-            const float4 v = a.sorted[valid ? j : gb];
-            const float xv[3] = {v.x, v.y, v.z};
-            bool well = valid;
+            __asm__ volatile("" ::: "memory");
+            int64_t *colq = a.col + bcast(mbase, m);
+            int64_t *rowq = a.row ? a.row + bcast(mbase, m) : nullptr;
+            const int64_t qloc = (int64_t)bcast(mloc, m);
+            for (int32_t k = lane; k < len; k += kWave) {
+                colq[k] = (int64_t)rowbuf[k];
+                if (rowq) rowq[k] = qloc;
+            }
+            __asm__ volatile("" ::: "memory");
+        }
+    }
+}
+
+// --------------------------------------------------------------- big cells --
+
+template <int DIM, int MODE, bool FILL>
+__device__ void group_big(const GroupArgs &a, uint32_t cid) {
+    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    const int lane = __lane_id();
+    const CellView v = make_view(a, load_cell(a, cid), kOff);
+    const CloudDev &cd = a.desc[v.c];
+    const Geo<DIM> g(cd, v.center);
+
+    for (int32_t cbase = 0; cbase < v.total; cbase += kBigChunk) {
+        const int nrounds = min(kBigRounds, (int)((v.total - cbase + kWave - 1) / kWave));
+        float cx[kBigRounds][3], cp[kBigRounds], chh[kBigRounds];
+        int32_t corig[kBigRounds];
+        Thr tj[kBigRounds];
+#pragma unroll
+        for (int r = 0; r < kBigRounds; ++r) {
+            const int32_t k = cbase + r * kWave + lane;
+            const bool valid = r < nrounds && k < v.total;
+            const int32_t j = candidate_slot(v, k, kOff);
+            corig[r] = (FILL && valid) ? a.sort_index[j] : 0;
+            const float4 cv = a.sorted[valid ? j : v.gb];
+            g.place(cv, valid, cx[r], cp[r]);
+            chh[r] = cv.w;
+            if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
+        }
+        for (uint32_t s = v.gb; s < v.ge; ++s) {
+            if (a.sflags[s]) continue;  // strays were listed by the small kernel
+            const int32_t qloc = uniform(a.sort_index[s]);
+            const int64_t qg = v.qs + qloc;
+            const float4 mq = a.sorted[s];
+            const float mqv[3] = {mq.x, mq.y, mq.z};
+            float xi2[3] = {0.f, 0.f, 0.f}, pi = 0.0f;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                float x = xv[d];
-                if ((pmask >> d) & 1u) {
-                    const float e0 = x - center[d];
-                    if (e0 > 0.5f * ext[d]) x = x - ext[d];
-                    else if (e0 < -0.5f * ext[d]) x = x + ext[d];
-                }
-                cx[r][d] = x;
-                well = well && fabsf(x - center[d]) <= box;
+                const float c = mqv[d] - g.center[d];
+                xi2[d] = -2.0f * c;
+                pi = d == 0 ? c * c : fmaf(c, c, pi);
             }
-            chh[r] = v.w;
-            vmask[r] = __ballot(valid);
-            exmask[r] = __ballot(valid && !well);
-        }
-        int32_t corig[kRounds];
-        if (FILL) {
-#pragma unroll
-            for (int r = 0; r < kRounds; ++r) corig[r] = a.sort_index[cj[r]];
-        }
-
-        // ---- loop over the group's queries
-        for (uint32_t s = gb; s < ge; ++s) {
-            const uint8_t fl = a.sflags[s];
-            const int32_t qloc = uniform(a.sort_index[s]);
-            const int64_t qg = qs + qloc;
-            if (fl) {
-                if (a.add_strays && cbase == 0 && lane == 0) {
-                    const uint32_t k = atomicAdd(a.stray_count, 1u);
-                    a.stray_list[k] = (int32_t)qg;
-                }
-                continue;
-            }
-            const float4 qv = a.sorted[s];
-            const float xi[3] = {uniform_f(qv.x), uniform_f(qv.y), uniform_f(qv.z)};
-            const float hi = uniform_f(qv.w);
+            const float hi = mq.w;
             Thr tq{0.f, 0.f};
-            if (MODE == TCR_MODE_GATHER) tq = thresholds(hi, delta, DIM);
+            if (MODE == TCR_MODE_GATHER) {
+                tq = thresholds(hi, g.geo, g.cell2, DIM);
+                tq.lo -= pi;
+                tq.hi -= pi;
+            }
             int32_t hits = 0;
             int64_t base = 0;
             if (FILL) base = a.offsets[qg] + (cbase ? (int64_t)a.cursor[qg] : 0);
 #pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
-                if (cbase + r * kWave >= total) break;
-                float s2 = 0.0f;
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    const float e = xi[d] - cx[r][d];
-                    s2 = d == 0 ? e * e : fmaf(e, e, s2);
-                }
-                Thr t;
-                if (MODE == TCR_MODE_GATHER) {
-                    t = tq;
-                } else if (MODE == TCR_MODE_SCATTER) {
-                    t = thresholds(chh[r], delta, DIM);
-                } else if (MODE == TCR_MODE_SYMMETRIC) {
-                    t = thresholds((hi + chh[r]) / 2.f, delta, DIM);
-                } else {
-                    t = thresholds(max_ref(hi, chh[r]), delta, DIM);
-                }
-                const uint64_t inm = __ballot(s2 < t.lo);
-                const uint64_t maybe = __ballot(s2 < t.hi);
-                uint64_t hitm = inm & vmask[r] & ~exmask[r];
-                const uint64_t band = ((maybe & ~inm) | exmask[r]) & vmask[r];
-                if (band) {
-                    bool ok = false;
-                    if ((band >> lane) & 1ull) {
-                        const float4 v = a.sorted[cj[r]];
-                        const float xj[3] = {v.x, v.y, v.z};
-                        const float dist = exact_distance<DIM>(xi, xj, mn, mx, pmask);
-                        ok = support_pred(dist, hi, v.w, MODE);
-                    }
-                    hitm |= __ballot(ok) & band;
-                }
+            for (int r = 0; r < kBigRounds; ++r) {
+                if (r >= nrounds) break;
+                const uint64_t hitm = round_hits<DIM, MODE>(
+                    a, cd, g, cx[r], cp[r], chh[r], tj[r], xi2, pi, hi, tq, s,
+                    [&]() { return candidate_slot(v, cbase + r * kWave + lane, kOff); });
                 if (FILL) {
                     if ((hitm >> lane) & 1ull) {
                         const int64_t pos = base + __popcll(hitm & lanemask_lt());
@@ -285,66 +628,176 @@ __device__ void group_one(const GroupArgs &a, uint32_t cid) {
                 hits += __popcll(hitm);
             }
             if (lane == 0) {
-                if (!FILL) {
-                    a.counts[qg] = cbase ? a.counts[qg] + hits : hits;
-                } else {
-                    a.cursor[qg] = (cbase ? a.cursor[qg] : 0) + hits;
-                }
+                if (!FILL) a.counts[qg] = cbase ? a.counts[qg] + hits : hits;
+                else a.cursor[qg] = (cbase ? a.cursor[qg] : 0) + hits;
             }
         }
     }
 }
 
-template <int DIM, int MODE, bool FILL>
-__global__ void __launch_bounds__(kGroupThreads) k_group(GroupArgs a) {
-    const uint32_t ncells = a.misc[0];
-    const int wave = threadIdx.x / kWave;
-    // XCD-aware split: workgroup b runs on XCD b % 8; each XCD takes a
-    // contiguous range of cells so that neighbouring cells share its L2.
+// ------------------------------------------------------------- kernels ----
+
+// XCD-aware static schedule: workgroup b runs on XCD b % 8; each XCD takes a
+// contiguous range of cells so that neighbouring cells share its L2.
+struct CellRange {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ CellRange my_cells(uint32_t ncells) {
     const uint32_t nxcd = 8;
+    const uint32_t wpb = kGroupThreads / kWave;
     const uint32_t xcd = blockIdx.x % nxcd;
-    const uint32_t per_xcd_waves = (gridDim.x / nxcd) * (kGroupThreads / kWave);
-    const uint32_t wix = (blockIdx.x / nxcd) * (kGroupThreads / kWave) + wave;
+    const uint32_t per_xcd_waves = (gridDim.x / nxcd) * wpb;
+    const uint32_t wix = (blockIdx.x / nxcd) * wpb + threadIdx.x / kWave;
+    CellRange r;
     const uint32_t c0 = (uint32_t)(((uint64_t)ncells * xcd) / nxcd);
-    const uint32_t c1 = (uint32_t)(((uint64_t)ncells * (xcd + 1)) / nxcd);
-    for (uint32_t cid = c0 + wix; cid < c1; cid += per_xcd_waves) {
-        const uint32_t cl = uniform(a.cell_cloud[cid]);
-        if (a.desc[cl].h.dim != DIM || a.desc[cl].h.mode != MODE || !a.desc[cl].h.self_search ||
-            a.desc[cl].h.search_range != 1)
-            continue;
-        group_one<DIM, MODE, FILL>(a, cid);
+    r.end = (uint32_t)(((uint64_t)ncells * (xcd + 1)) / nxcd);
+    r.first = c0 + wix;
+    r.step = per_xcd_waves;
+    return r;
+}
+
+__device__ __forceinline__ bool cloud_is(const CloudDev &cd, int dim, int mode) {
+    return cd.h.dim == dim && (mode < 0 || cd.h.mode == mode);
+}
+
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
+    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
+    const CellRange cr = my_cells(a.misc[0]);
+    if (cr.first >= cr.end) return;
+    CellLoad cur = load_cell(a, cr.first);
+    for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
+        const uint32_t nid = cid + cr.step;
+        CellLoad nxt = cur;
+        if (nid < cr.end) nxt = load_cell(a, nid);  // in flight during this cell
+        const CellView v = make_view(a, cur, kOff);
+        if (v.state != kCellSkip && cloud_is(a.desc[v.c], DIM, MODE)) {
+            if (v.state == kCellStray) {
+                append_strays(a, v);
+            } else if (v.total > kSmallChunk) {
+                if (__lane_id() == 0) a.big_list[atomicAdd(a.big_count, 1u)] = cid;
+            } else {
+                count_small<DIM, MODE>(a, v, cid, s_slot[threadIdx.x / kWave]);
+            }
+        }
+        cur = nxt;
     }
 }
 
-template <bool FILL>
-int launch_group_t(const Batch &b, const GroupArgs &ga, hipStream_t s) {
-    // workgroups: a multiple of 8 (XCDs), up to 8 wavefront groups per CU
-    int64_t waves = b.nr_total;  // upper bound on occupied cells
-    int64_t wgs = (waves + 3) / 4;
+__global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
+    __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
+    __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
+    const CellRange cr = my_cells(a.misc[0]);
+    if (cr.first >= cr.end) return;
+    CellLoad cur = load_cell(a, cr.first);
+    for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
+        const uint32_t nid = cid + cr.step;
+        CellLoad nxt = cur;
+        if (nid < cr.end) nxt = load_cell(a, nid);
+        const uint32_t state = uniform(cur.rec.state);
+        if (state == kCellFast) {
+            const int dim = a.desc[uniform(cur.rec.cloud)].h.dim;
+            const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
+            const CellView v = make_view(a, cur, noff);
+            if (v.total <= kSmallChunk)  // bigger cells: k_group_big
+                fill_small(a, v, noff, s_slot[threadIdx.x / kWave], s_row[threadIdx.x / kWave]);
+        }
+        cur = nxt;
+    }
+}
+
+template <int DIM, int MODE, bool FILL>
+__global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a) {
+    const uint32_t n = *a.big_count;
+    const uint32_t wpb = kGroupThreads / kWave;
+    for (uint32_t i = blockIdx.x * wpb + threadIdx.x / kWave; i < n; i += gridDim.x * wpb) {
+        const uint32_t cid = a.big_list[i];
+        const CloudDev &cd = a.desc[uniform(a.cell_cloud[cid])];
+        if (!cloud_is(cd, DIM, MODE)) continue;
+        group_big<DIM, MODE, FILL>(a, cid);
+    }
+}
+
+// Row offset of each sorted slot's query (self-search clouds): the fill's
+// per-query base without a dependent gather inside the kernel.
+__global__ void __launch_bounds__(256) k_sorted_offsets(const CloudDev *__restrict__ desc, int32_t B,
+                                                        const int64_t *__restrict__ r_start,
+                                                        const int64_t *__restrict__ q_start, int64_t n,
+                                                        const int32_t *__restrict__ sort_index,
+                                                        const int64_t *__restrict__ offsets,
+                                                        int64_t *__restrict__ soff) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int c = find_segment(B, s, [&](int k) { return r_start[k]; });
+    if (!group_owned(desc[c])) return;
+    soff[s] = offsets[q_start[c] + sort_index[s]];
+}
+
+// ------------------------------------------------------------- driver -----
+
+static unsigned group_grid(const Batch &b) {
+    int64_t wgs = (b.nr_total + 3) / 4;   // upper bound on occupied cells / 4
     if (wgs > 256 * 8) wgs = 256 * 8;
     wgs = ((wgs + 7) / 8) * 8;
-    const dim3 grid((unsigned)wgs), block(kGroupThreads);
+    return (unsigned)(wgs > 0 ? wgs : 8);
+}
+
+template <bool FILL>
+static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
     bool seen[4][4] = {};
-    for (const CloudDev &d : b.dev) {
-        if (!d.h.self_search || d.h.search_range != 1) continue;
-        seen[d.h.dim][d.h.mode] = true;
+    for (const CloudDev &d : b.dev)
+        if (d.h.self_search && d.h.search_range == 1 && d.h.nq > 0) seen[d.h.dim][d.h.mode] = true;
+    const dim3 grid(group_grid(b)), block(kGroupThreads), bgrid(256);
+#define TCR_L(D, M)                                                                          \
+    if (seen[D][M]) {                                                                        \
+        if (!FILL) hipLaunchKernelGGL((k_count_small<D, M>), grid, block, 0, s, ga);         \
+        hipLaunchKernelGGL((k_group_big<D, M, FILL>), bgrid, block, 0, s, ga);               \
     }
-#define TCR_LAUNCH_GROUP(D, M)                                                             \
-    if (seen[D][M]) hipLaunchKernelGGL((k_group<D, M, FILL>), grid, block, 0, s, ga);
-#define TCR_LAUNCH_GROUP_DIM(D)                                                            \
-    TCR_LAUNCH_GROUP(D, TCR_MODE_SYMMETRIC) TCR_LAUNCH_GROUP(D, TCR_MODE_GATHER)          \
-    TCR_LAUNCH_GROUP(D, TCR_MODE_SCATTER) TCR_LAUNCH_GROUP(D, TCR_MODE_SUPERSYMMETRIC)
-    TCR_LAUNCH_GROUP_DIM(1)
-    TCR_LAUNCH_GROUP_DIM(2)
-    TCR_LAUNCH_GROUP_DIM(3)
-#undef TCR_LAUNCH_GROUP_DIM
-#undef TCR_LAUNCH_GROUP
+#define TCR_LD(D) TCR_L(D, 0) TCR_L(D, 1) TCR_L(D, 2) TCR_L(D, 3)
+    if (!FILL) {
+        TCR_LD(1) TCR_LD(2) TCR_LD(3)
+    } else {
+        hipLaunchKernelGGL(k_fill_small, grid, block, 0, s, ga);
+        TCR_LD(1) TCR_LD(2) TCR_LD(3)
+    }
+#undef TCR_LD
+#undef TCR_L
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
 
 int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
                    bool list, hipStream_t s);
+
+static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64_t *col) {
+    GroupArgs ga;
+    ga.desc = w.desc;
+    ga.r_start = w.r_start;
+    ga.q_start = w.q_start;
+    ga.misc = w.misc;
+    ga.cell_begin = w.cell_begin;
+    ga.cell_cloud = w.cell_cloud;
+    ga.hash_table = w.hash_table;
+    ga.cell_table = w.cell_table;
+    ga.sorted = w.sorted;
+    ga.sort_index = w.sort_index;
+    ga.sflags = w.sflags;
+    ga.ranges = w.ranges;
+    ga.recs = (CellRec *)w.recs;
+    ga.masks = w.masks;
+    ga.counts = w.counts;
+    ga.offsets = offsets;
+    ga.soff = w.soff;
+    ga.row = row;
+    ga.col = col;
+    ga.cursor = w.cursor;
+    ga.stray_list = w.stray_list;
+    ga.stray_count = w.misc + 1;
+    ga.big_list = w.big_list;
+    ga.big_count = w.misc + 2;
+    return ga;
+}
 
 // Count or fill pass over the batch: wavefront-per-cell kernels for the
 // self-search clouds (their strays collected by the count pass), the
@@ -359,28 +812,17 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
         else any_other = true;
     }
     if (any_group) {
-        if (!fill) TCR_HIP(hipMemsetAsync(w.misc + 1, 0, sizeof(uint32_t), s));
-        GroupArgs ga;
-        ga.desc = w.desc;
-        ga.r_start = w.r_start;
-        ga.q_start = w.q_start;
-        ga.misc = w.misc;
-        ga.cell_begin = w.cell_begin;
-        ga.cell_cloud = w.cell_cloud;
-        ga.hash_table = w.hash_table;
-        ga.cell_table = w.cell_table;
-        ga.sorted = w.sorted;
-        ga.sort_index = w.sort_index;
-        ga.sflags = w.sflags;
-        ga.counts = w.counts;
-        ga.offsets = offsets;
-        ga.row = row;
-        ga.col = col;
-        ga.cursor = w.cursor;
-        ga.stray_list = w.stray_list;
-        ga.stray_count = w.misc + 1;
-        ga.add_strays = fill ? 0 : 1;
-        int rc = fill ? launch_group_t<true>(b, ga, s) : launch_group_t<false>(b, ga, s);
+        const GroupArgs ga = group_args(w, offsets, row, col);
+        if (!fill) {
+            TCR_HIP(hipMemsetAsync(w.misc + 1, 0, 2 * sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_stencil, dim3(2048), dim3(256), 0, s, ga);
+        } else {
+            hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((b.nr_total + 255) / 256)), dim3(256), 0, s,
+                               (const CloudDev *)w.desc, b.B, (const int64_t *)w.r_start,
+                               (const int64_t *)w.q_start, b.nr_total, (const int32_t *)w.sort_index, offsets,
+                               w.soff);
+        }
+        int rc = fill ? launch_modes<true>(b, ga, s) : launch_modes<false>(b, ga, s);
         if (rc) return rc;
         rc = launch_generic(b, w, offsets, row, col, fill, true, s);
         if (rc) return rc;

@@ -100,6 +100,11 @@ struct Work {
     uint8_t *sflags;      // [Nr] stray bits per sorted slot
     int32_t *cursor;      // [Nq] per-query write cursor across chunks
     int32_t *stray_list;  // [Nq] global query indices for the per-query path
+    uint64_t *masks;      // [Nr * 7] per sorted slot: hit mask of each candidate round
+    int2 *ranges;         // [Nr * 27] per cell: stencil cell (first global slot, count)
+    float4 *recs;         // [Nr * 2] per cell: 32-byte record (slots, cloud, state, centre)
+    int64_t *soff;        // [Nr] row offset of each sorted slot's query
+    uint32_t *big_list;   // [Nr] cells with more candidates than one register chunk
 };
 
 // One cloud's grid and tables as the traversal reads them.
