@@ -10,14 +10,18 @@ namespace tcr {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-int build_structure(const Batch &b, Work &w, hipStream_t s);
 int query_count(const Batch &b, Work &w, hipStream_t s);
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
-                       hipStream_t s);
-struct GridView;
-struct RefOpArgs;
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, hipStream_t s);
 
 static int64_t tiles_of(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+// Slots of a cloud's lin -> cell table: a power of two at least twice the
+// number of reference points (load factor <= 1/2).
+static int64_t lintab_slots(int64_t nr) {
+    int64_t cap = 16;
+    while (cap < 2 * nr) cap <<= 1;
+    return cap;
+}
 
 bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) {
     if (n < 1 || !clouds) {
@@ -27,7 +31,6 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) 
     b = Batch();
     b.B = n;
     b.dev.resize(n);
-    b.rt_prefix.assign(n + 1, 0);
     for (int32_t c = 0; c < n; ++c) {
         const tcr_cloud &h = clouds[c];
         if (h.dim < 1 || h.dim > 3) { err = "dim must be 1, 2 or 3 (cloud " + std::to_string(c) + ")"; return false; }
@@ -40,13 +43,14 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) 
         std::memset(&d, 0, sizeof(d));
         d.h = h;
         if (h.self_search) d.h.nr = h.nq;
-        b.max_dim = h.dim > b.max_dim ? h.dim : b.max_dim;
         d.rt_off = b.rt_total;
         d.rt_n = tiles_of(d.h.nr);
         b.rt_total += d.rt_n;
-        b.rt_prefix[c + 1] = b.rt_total;
         d.hb_off = b.hb_total;
         b.hb_total += h.hash_map_length;
+        d.lt_off = b.lt_total;
+        d.lt_cap = lintab_slots(d.h.nr);
+        b.lt_total += d.lt_cap;
         b.nq_total += h.nq;
         b.nr_total += d.h.nr;
         if (!h.self_search) b.any_nonself = true;
@@ -63,42 +67,48 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) 
     return true;
 }
 
+// Bytes of the header block: the descriptors followed by the five prefix
+// arrays (q_start, r_start, rt_start, hb_start, lt_start), contiguous so that
+// one host-to-device copy uploads them.
+static size_t header_bytes(int64_t B) { return sizeof(CloudDev) * B + 5 * sizeof(int64_t) * (B + 1); }
+
+static void header_views(char *base, int64_t B, Work &w) {
+    w.desc = (CloudDev *)base;
+    int64_t *pre = base ? (int64_t *)(base + sizeof(CloudDev) * B) : nullptr;
+    int64_t **dst[5] = {&w.q_start, &w.r_start, &w.rt_start, &w.hb_start, &w.lt_start};
+    for (int k = 0; k < 5; ++k) *dst[k] = pre ? pre + k * (B + 1) : nullptr;
+}
+
 size_t carve_work(const Batch &b, void *ws, Work &w) {
+    static_assert(sizeof(CloudDev) % sizeof(int64_t) == 0, "descriptor block keeps the prefixes aligned");
     Carve cv(ws);
     const int64_t B = b.B, nr = b.nr_total, nq = b.nq_total, hb = b.hb_total;
-    w.desc = cv.take<CloudDev>(B);
-    w.q_start = cv.take<int64_t>(B + 1);
-    w.r_start = cv.take<int64_t>(B + 1);
-    w.rt_start = cv.take<int64_t>(B + 1);
-    w.qt_start = cv.take<int64_t>(B + 1);
-    w.hb_start = cv.take<int64_t>(B + 1);
+    header_views(cv.take<char>(header_bytes(B)), B, w);
+    w.misc = cv.take<uint32_t>(kMiscWords);
     w.rwrap = cv.take<float4>(nr);
     w.keys0 = cv.take<uint32_t>(nr);
     w.keys1 = cv.take<uint32_t>(nr);
     w.vals0 = cv.take<uint32_t>(nr);
     w.vals1 = cv.take<uint32_t>(nr);
-    w.hist = cv.take<uint32_t>((size_t)kRadix * b.rt_total);
-    int64_t scan_max = (int64_t)kRadix * b.rt_total;
-    scan_max = scan_max > nr ? scan_max : nr;
-    scan_max = scan_max > nq ? scan_max : nq;
-    scan_max = scan_max > hb ? scan_max : hb;
+    w.sort_hist = cv.take<uint32_t>((size_t)B * kMaxPasses * kRadix);
+    w.sort_lb = cv.take<uint64_t>((size_t)kMaxPasses * b.rt_total * kRadix);
+    w.cell_lb = cv.take<uint64_t>(b.rt_total);
+    int64_t scan_max = nq > hb ? nq : hb;
     w.scan_part = cv.take<uint64_t>(scan_partials_needed(scan_max));
     w.sorted = cv.take<float4>(nr);
     w.sort_index = cv.take<int32_t>(nr);
     w.skey = cv.take<int32_t>(nr);
-    w.flags = cv.take<uint32_t>(nr);
-    w.cellpos = cv.take<uint32_t>(nr);
     w.cell_begin = cv.take<uint32_t>(nr + 1);
     w.cell_lin = cv.take<int32_t>(nr);
     w.cell_hash = cv.take<uint32_t>(nr);
     w.cell_cloud = cv.take<uint32_t>(nr);
+    w.lintab = cv.take<uint64_t>(b.lt_total);
     w.bucket_cnt = cv.take<uint32_t>(hb);
     w.bucket_begin = cv.take<uint32_t>(hb);
     w.bucket_fill = cv.take<uint32_t>(hb);
     w.cell_table = cv.take<int32_t>(3 * (size_t)nr);
     w.hash_table = cv.take<int32_t>(2 * (size_t)hb);
     w.counts = cv.take<int32_t>(nq);
-    w.misc = cv.take<uint32_t>(64);
     w.pflags = cv.take<uint8_t>(nr);
     w.sflags = cv.take<uint8_t>(nr);
     w.cursor = cv.take<int32_t>(nq);
@@ -114,7 +124,7 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
 struct Prepared {
     Batch b;
     Work w;
-    std::vector<int64_t> pre;  // host prefixes; alive until the caller's sync
+    std::vector<char> header;  // host image of the header block
 };
 
 // Validates, carves the workspace and uploads descriptors and prefixes.
@@ -132,8 +142,10 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
         return TCR_ERR_SPACE;
     }
     Batch &b = p.b;
-    p.pre.assign(5 * (size_t)(b.B + 1), 0);
-    int64_t *qs = p.pre.data(), *rs = qs + b.B + 1, *rts = rs + b.B + 1, *qts = rts + b.B + 1, *hbs = qts + b.B + 1;
+    p.header.assign(header_bytes(b.B), 0);
+    Work hv;
+    header_views(p.header.data(), b.B, hv);
+    int64_t *qs = hv.q_start, *rs = hv.r_start, *rts = hv.rt_start, *hbs = hv.hb_start, *lts = hv.lt_start;
     for (int32_t c = 0; c < b.B; ++c) {
         CloudDev &d = b.dev[c];
         const tcr_cloud &h = clouds[c];
@@ -154,16 +166,13 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
         qs[c + 1] = qs[c] + h.nq;
         rs[c + 1] = rs[c] + d.h.nr;
         rts[c + 1] = rts[c] + d.rt_n;
-        qts[c + 1] = qts[c] + d.qt_n;
         hbs[c + 1] = hbs[c] + h.hash_map_length;
+        lts[c + 1] = lts[c] + d.lt_cap;
     }
     if (upload) {
-        TCR_HIP(hipMemcpyAsync(p.w.desc, b.dev.data(), sizeof(CloudDev) * b.B, hipMemcpyHostToDevice, s));
-        TCR_HIP(hipMemcpyAsync(p.w.q_start, qs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
-        TCR_HIP(hipMemcpyAsync(p.w.r_start, rs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
-        TCR_HIP(hipMemcpyAsync(p.w.rt_start, rts, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
-        TCR_HIP(hipMemcpyAsync(p.w.qt_start, qts, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
-        TCR_HIP(hipMemcpyAsync(p.w.hb_start, hbs, sizeof(int64_t) * (b.B + 1), hipMemcpyHostToDevice, s));
+        std::memcpy(hv.desc, b.dev.data(), sizeof(CloudDev) * b.B);
+        // the host image outlives the stream synchronisation of tcr_search_count
+        TCR_HIP(hipMemcpyAsync(p.w.desc, p.header.data(), p.header.size(), hipMemcpyHostToDevice, s));
     }
     return TCR_OK;
 }
@@ -274,6 +283,11 @@ int tcr_search_export(const tcr_cloud *clouds, int32_t n_clouds, int32_t cloud, 
         if (sorted_pos || sorted_sup)
             hipLaunchKernelGGL(k_export_sorted, dim3(g), dim3(256), 0, s, (const float4 *)p.w.sorted + r0, nr,
                                hd.h.dim, sorted_pos, sorted_sup);
+    }
+    if (hash_table || cell_table) {
+        // the search itself never builds the bucket layout
+        rc = build_bucket_tables(p.b, p.w, s);
+        if (rc) return rc;
     }
     if (hash_table)
         TCR_HIP(hipMemcpyAsync(hash_table, p.w.hash_table + 2 * hd.hb_off,

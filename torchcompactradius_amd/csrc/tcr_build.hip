@@ -1,7 +1,9 @@
-// Structure build on the device: hMax, hCell, wrapped cell keys, sorted
-// positions, cell table and compact hash table -- the GPU form of
-// neighborSearch_cpp's preamble (py/compactHashing/cppSearch.py:215-224) and
-// buildCompactHashMap_compat (py/compactHashing/hashTable.py:79-154).
+// Structure build on the device: hMax, hCell, wrapped cell keys, the sort
+// into cell order, the cell table and the lin -> cell table -- the GPU form
+// of neighborSearch_cpp's preamble (py/compactHashing/cppSearch.py:215-224)
+// and buildCompactHashMap_compat (py/compactHashing/hashTable.py:79-154).
+// The reference's bucket-ordered tables (hashTable [L,2], sortedCellTable)
+// are built only when a structure is exported (build_bucket_tables).
 #include <algorithm>
 
 #include "tcr_internal.h"
@@ -14,58 +16,23 @@ __device__ __forceinline__ int cloud_of(int32_t B, int64_t g, const int64_t *__r
     return find_segment(B, g, [&](int k) { return start[k]; });
 }
 
-// hMax per cloud (computeGridSupport, py/compactHashing/cellTable.py:40-81):
-// gather reads query supports, scatter reference supports, the symmetric
-// modes both.  which = 0: query set, 1: reference set.
-// Each workgroup reduces one contiguous slice of the concatenated supports;
-// a thread keeps a running maximum for its current cloud and flushes it with
-// one atomicMax when the cloud changes, and the wavefront merges lanes that
-// end on the same cloud -- a few thousand atomics for any batch.
-constexpr int kHmaxBlocks = 512;
-__global__ void __launch_bounds__(kBlock) k_hmax(CloudDev *__restrict__ desc, int32_t B,
-                                                 const int64_t *__restrict__ start, int64_t n, int which) {
-    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
-    const int64_t g0 = (int64_t)blockIdx.x * per, g1 = min(n, g0 + per);
-    int c = -1;
-    uint32_t m = 0u;  // ordered encoding: 0 is below every float
-    bool use = false;
-    const float *sup = nullptr;
-    int64_t c_lo = 0, c_hi = -1;  // [start[c], start[c+1])
-    for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
-        if (g >= c_hi) {
-            if (c >= 0 && m) atomicMax(&desc[c].hmax_ord, m);
-            c = cloud_of(B, g, start);
-            c_lo = start[c];
-            c_hi = start[c + 1];
-            m = 0u;
-            const CloudDev &cd = desc[c];
-            const int mode = cd.h.mode;
-            // a self-search cloud's query set is its reference set
-            use = which == 0 ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
-                             : (mode != TCR_MODE_GATHER && !cd.h.self_search);
-            sup = which == 0 ? cd.qs : cd.rs;
-            if (!sup) use = false;
-        }
-        if (use) m = max(m, float_to_ordered(sup[g - c_lo]));
-    }
-    const int c0 = __shfl(c, 0, kWave);
-    if (__all(c == c0)) {
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
-        if ((threadIdx.x & (kWave - 1)) == 0 && c0 >= 0 && m) atomicMax(&desc[c0].hmax_ord, m);
-    } else if (c >= 0 && m) {
-        atomicMax(&desc[c].hmax_ord, m);
-    }
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, uint32_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = v;
+}
+
+int fill_u32(void *p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return TCR_OK;
+    const int64_t blocks = std::min<int64_t>(2048, ((int64_t)n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(kBlock), 0, s, (uint32_t *)p, v, (int64_t)n);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
 }
 
 // hCell = compute_h(min, max, hMax) (py/util.py:28-79) in fp32, numCells =
 // ceil(extent / hCell).int32 (py/compactHashing/cellTable.py:29).
-__global__ void k_grid(CloudDev *__restrict__ desc, int32_t B) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= B) return;
-    CloudDev &cd = desc[c];
+__device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
     const int dim = cd.h.dim;
-    const float hmax = cd.hmax_ord ? ordered_to_float(cd.hmax_ord) : -INFINITY;
+    const float hmax = hmax_ord ? ordered_to_float(hmax_ord) : -INFINITY;
     float ext[3] = {0.f, 0.f, 0.f}, nc[3] = {0.f, 0.f, 0.f}, h[3] = {0.f, 0.f, 0.f};
     bool over = false, stretch = false;
     for (int d = 0; d < dim; ++d) {
@@ -106,13 +73,92 @@ __global__ void k_grid(CloudDev *__restrict__ desc, int32_t B) {
     cd.fast = fast;
 }
 
+// hMax per cloud (computeGridSupport, py/compactHashing/cellTable.py:40-81):
+// gather reads query supports, scatter reference supports, the symmetric
+// modes both.  Blocks [0, nbq) reduce slices of the concatenated query
+// supports, the rest slices of the reference supports; a block reduces per
+// cloud in registers and LDS and issues one atomicMax per cloud it touched.
+// The last block to finish computes every cloud's grid.
+__global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, int32_t B,
+                                                 const int64_t *__restrict__ q_start, int64_t nq, int nbq,
+                                                 const int64_t *__restrict__ r_start, int64_t nr, int nbr,
+                                                 uint32_t *__restrict__ misc) {
+    __shared__ uint32_t s_max[kBlock / kWave];
+    __shared__ int s_cloud[kBlock / kWave];
+    __shared__ bool s_last;
+    const bool qside = (int)blockIdx.x < nbq;
+    const int bi = qside ? (int)blockIdx.x : (int)blockIdx.x - nbq;
+    const int nb = qside ? nbq : nbr;
+    const int64_t n = qside ? nq : nr;
+    const int64_t *start = qside ? q_start : r_start;
+    if (nb > 0 && n > 0) {
+        const int64_t per = ((n + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
+        const int64_t g0 = (int64_t)bi * per, g1 = min(n, g0 + per);
+        int c = -1;
+        uint32_t m = 0u;  // ordered encoding: 0 is below every float
+        bool use = false;
+        const float *sup = nullptr;
+        int64_t c_lo = 0, c_hi = -1;
+        for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
+            if (g >= c_hi) {
+                if (c >= 0 && m) atomicMax(&desc[c].hmax_ord, m);
+                c = cloud_of(B, g, start);
+                c_lo = start[c];
+                c_hi = start[c + 1];
+                m = 0u;
+                const CloudDev &cd = desc[c];
+                const int mode = cd.h.mode;
+                // a self-search cloud's query set is its reference set
+                use = qside ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
+                            : (mode != TCR_MODE_GATHER && !cd.h.self_search);
+                sup = qside ? cd.qs : cd.rs;
+                if (!sup) use = false;
+            }
+            if (use) m = max(m, float_to_ordered(sup[g - c_lo]));
+        }
+        // lanes ending on the same cloud as lane 0 merge in the wave, then
+        // in the block; others flush alone
+        const int c0 = __shfl(c, 0, kWave);
+        if (c != c0) {
+            if (c >= 0 && m) atomicMax(&desc[c].hmax_ord, m);
+            m = 0u;
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            s_max[threadIdx.x / kWave] = m;
+            s_cloud[threadIdx.x / kWave] = c0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 0; w < kBlock / kWave; ++w) {
+                uint32_t mm = s_max[w];
+                const int cw = s_cloud[w];
+                if (cw < 0 || !mm) continue;
+                for (int w2 = w + 1; w2 < kBlock / kWave; ++w2)
+                    if (s_cloud[w2] == cw) {
+                        mm = max(mm, s_max[w2]);
+                        s_max[w2] = 0;
+                    }
+                atomicMax(&desc[cw].hmax_ord, mm);
+            }
+        }
+    }
+    // last block: grids of every cloud
+    __threadfence();
+    if (threadIdx.x == 0) s_last = atomicAdd(&misc[kMiscPrepTicket], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int c = threadIdx.x; c < B; c += kBlock)
+        grid_of(desc[c], __hip_atomic_load(&desc[c].hmax_ord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // Reference set: second wrap (cppSearch.py:221) on top of radiusSearch's
 // first (neighborhood.py:203), cell coordinates and int32 linear key
-// (cellTable.py:30-31).  Keys are stored sign-flipped so that an unsigned
-// radix sort yields the signed order of torch.argsort.
-// Stray bits for the wavefront-per-cell path: bit 0 -- the second wrap moved
-// the point (its query copy, wrapped once, differs from its sorted copy);
-// bit 1 -- its cell lies outside [0, numCells) in some dimension.
+// (cellTable.py:30-31).  Stray bits for the wavefront-per-cell path: bit 0
+// -- the second wrap moved the point (its query copy, wrapped once, differs
+// from its sorted copy); bit 1 -- its cell lies outside [0, numCells).
 template <int DIM>
 __device__ __forceinline__ void ref_key_one(const CloudDev &cd, int64_t local, float4 &wp, int32_t &lin,
                                             uint8_t &flags) {
@@ -138,58 +184,47 @@ __device__ __forceinline__ void ref_key_one(const CloudDev &cd, int64_t local, f
     flags = f;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ref_keys(const CloudDev *__restrict__ desc, int32_t B,
-                                                     const int64_t *__restrict__ r_start, int64_t n,
-                                                     float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
-                                                     uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags) {
+__global__ void __launch_bounds__(kBlock) k_keys(CloudDev *__restrict__ desc, int32_t B,
+                                                 const int64_t *__restrict__ r_start, int64_t n,
+                                                 float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= n) return;
-    const int c = cloud_of(B, g, r_start);
-    const CloudDev &cd = desc[c];
-    const int64_t local = g - r_start[c];
-    float4 wp;
-    int32_t lin;
-    uint8_t f;
-    switch (cd.h.dim) {
-        case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
-        case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
-        default: ref_key_one<3>(cd, local, wp, lin, f); break;
+    int c = -1;
+    bool out = false;
+    if (g < n) {
+        c = cloud_of(B, g, r_start);
+        const CloudDev &cd = desc[c];
+        const int64_t local = g - r_start[c];
+        float4 wp;
+        int32_t lin;
+        uint8_t f;
+        switch (cd.h.dim) {
+            case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
+            case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
+            default: ref_key_one<3>(cd, local, wp, lin, f); break;
+        }
+        rwrap[g] = wp;
+        keys[g] = (uint32_t)lin;
+        vals[g] = (uint32_t)local;
+        pflags[g] = f;
+        out = (f & 2) != 0;
     }
-    rwrap[g] = wp;
-    keys[g] = ((uint32_t)lin) ^ 0x80000000u;
-    vals[g] = (uint32_t)local;
-    pflags[g] = f;
+    // the cloud is irregular if any point's cell is outside the grid
+    const int c0 = __shfl(c, 0, kWave);
+    if (__all(c == c0)) {
+        if (__any(out) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(&desc[c0].irregular, 1u);
+    } else if (out) {
+        atomicOr(&desc[c].irregular, 1u);
+    }
 }
 
-// sortedPositions = x[sortIndex] (hashTable.py:109), sortedSupports
-// (cppSearch.py:224), sorted keys, and the cell-start flags of
-// unique_consecutive (hashTable.py:113).
-__global__ void __launch_bounds__(kBlock) k_gather_sorted(const CloudDev *__restrict__ desc, int32_t B,
-                                                          const int64_t *__restrict__ r_start, int64_t n,
-                                                          const uint32_t *__restrict__ keys,
-                                                          const uint32_t *__restrict__ vals,
-                                                          const float4 *__restrict__ rwrap,
-                                                          float4 *__restrict__ sorted, int32_t *__restrict__ sidx,
-                                                          int32_t *__restrict__ skey, uint32_t *__restrict__ flags,
-                                                          const uint8_t *__restrict__ pflags,
-                                                          uint8_t *__restrict__ sflags) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const int c = cloud_of(B, s, r_start);
-    const uint32_t v = vals[s];
-    const int32_t k = (int32_t)(keys[s] ^ 0x80000000u);
-    sorted[s] = rwrap[r_start[c] + v];
-    sflags[s] = pflags[r_start[c] + v];
-    sidx[s] = (int32_t)v;
-    skey[s] = k;
-    const bool first = (s == r_start[c]) || ((int32_t)(keys[s - 1] ^ 0x80000000u) != k);
-    flags[s] = first ? 1u : 0u;
-}
-
-// One record per occupied cell: first slot, linear index, and the hash of
-// the first point's cell coordinates (hashTable.py:121-136).
+// Cell extraction (unique_consecutive of hashTable.py:113 with the cell
+// coordinates of its first point, :121-122, and their bucket, :136):
+// one tile of sorted slots per block, cell numbers by decoupled look-back
+// over all earlier tiles (cells are numbered across the whole batch), and
+// every cell inserted into its cloud's lin -> cell table.
 template <int DIM>
-__device__ __forceinline__ uint32_t cell_hash_of(const CloudDev &cd, float4 p) {
+__device__ __forceinline__ uint32_t cell_bucket(const CloudDev &cd, float4 p) {
     const float x[3] = {p.x, p.y, p.z};
     int32_t c[3];
 #pragma unroll
@@ -197,52 +232,183 @@ __device__ __forceinline__ uint32_t cell_hash_of(const CloudDev &cd, float4 p) {
     return hash_index<DIM>(c, cd.h.hash_map_length);
 }
 
-__global__ void __launch_bounds__(kBlock) k_cell_emit(const CloudDev *__restrict__ desc, int32_t B,
-                                                      const int64_t *__restrict__ r_start, int64_t n,
-                                                      const uint32_t *__restrict__ flags,
-                                                      const uint32_t *__restrict__ cellpos,
-                                                      const int32_t *__restrict__ skey,
-                                                      const float4 *__restrict__ sorted,
-                                                      uint32_t *__restrict__ cell_begin, int32_t *__restrict__ cell_lin,
-                                                      uint32_t *__restrict__ cell_hash,
-                                                      uint32_t *__restrict__ cell_cloud, uint32_t *__restrict__ misc) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    if (s == n - 1) {
-        const uint32_t total = cellpos[s] + flags[s];
-        cell_begin[total] = (uint32_t)n;
-        misc[0] = total;
+struct CellsArgs {
+    CloudDev *desc;
+    int32_t B;
+    int64_t ntiles, n;
+    const int64_t *t_start, *p_start;
+    const int32_t *skey;
+    const float4 *sorted;
+    uint32_t *cell_begin;
+    int32_t *cell_lin;
+    uint32_t *cell_hash;
+    uint32_t *cell_cloud;
+    uint64_t *lintab;
+    uint64_t *lb;
+    uint32_t *misc;
+};
+
+constexpr uint64_t kCellAgg = 1ull << 32;
+constexpr uint64_t kCellInc = 2ull << 32;
+
+__global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
+    constexpr int kItems = kSortTile / kSortThreads;  // 16 consecutive slots per thread
+    __shared__ uint32_t s_wsum[kSortThreads / kWave];
+    __shared__ int64_t s_tile;
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&a.misc[kMiscCellTicket], 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int c = find_segment(a.B, tile, [&](int k) { return a.t_start[k]; });
+    const int64_t first_tile = a.t_start[c];
+    const int64_t cstart = a.p_start[c], cn = a.p_start[c + 1] - cstart;
+    const int64_t t0 = cstart + (tile - first_tile) * kSortTile;
+    const int64_t len = min((int64_t)kSortTile, cn - (tile - first_tile) * kSortTile);
+    // flags of this thread's slots
+    const int64_t s0 = t0 + (int64_t)threadIdx.x * kItems;
+    uint32_t fl = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t s = s0 + k;
+        if (s - t0 < len) {
+            const bool f = (s == cstart) || (a.skey[s] != a.skey[s - 1]);
+            fl |= (uint32_t)f << k;
+            cnt += f;
+        }
     }
-    if (!flags[s]) return;
-    const int c = cloud_of(B, s, r_start);
-    const CloudDev &cd = desc[c];
-    const uint32_t cid = cellpos[s];
-    cell_begin[cid] = (uint32_t)s;
-    cell_lin[cid] = skey[s];
-    cell_cloud[cid] = (uint32_t)c;
-    uint32_t h;
-    switch (cd.h.dim) {
-        case 1: h = cell_hash_of<1>(cd, sorted[s]); break;
-        case 2: h = cell_hash_of<2>(cd, sorted[s]); break;
-        default: h = cell_hash_of<3>(cd, sorted[s]); break;
+    // block scan of the per-thread counts
+    uint32_t incl = cnt;
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += v;
     }
-    cell_hash[cid] = h;
+    if (lane == kWave - 1) s_wsum[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    uint32_t wpre = 0, ttot = 0;
+    for (int w = 0; w < kSortThreads / kWave; ++w) {
+        if (w < (int)threadIdx.x / kWave) wpre += s_wsum[w];
+        ttot += s_wsum[w];
+    }
+    // look-back over all earlier tiles of the batch
+    if (threadIdx.x == 0) {
+        uint64_t *lb = a.lb + tile;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(lb, kCellInc | ttot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(lb, kCellAgg | ttot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t t = tile - 1;
+            while (true) {
+                const uint64_t v = __hip_atomic_load(a.lb + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t flag = v & ~0xffffffffull;
+                if (flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)v;
+                if (flag == kCellInc) break;
+                --t;
+            }
+            __hip_atomic_store(lb, kCellInc | (excl + ttot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        if (tile == first_tile) a.desc[c].cell_off = (int64_t)excl;
+        if (tile == a.ntiles - 1) {
+            a.misc[kMiscCells] = (uint32_t)(excl + ttot);
+            a.cell_begin[excl + ttot] = (uint32_t)a.n;
+        }
+    }
+    __syncthreads();
+    uint32_t cid = (uint32_t)(s_excl + wpre + incl - cnt);
+    const CloudDev &cd = a.desc[c];
+    uint64_t *tab = a.lintab + cd.lt_off;
+    const uint64_t cap = (uint64_t)cd.lt_cap;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (!((fl >> k) & 1u)) continue;
+        const int64_t s = s0 + k;
+        const int32_t lin = a.skey[s];
+        a.cell_begin[cid] = (uint32_t)s;
+        a.cell_lin[cid] = lin;
+        a.cell_cloud[cid] = (uint32_t)c;
+        const float4 p = a.sorted[s];
+        a.cell_hash[cid] = cd.h.dim == 1 ? cell_bucket<1>(cd, p) : cd.h.dim == 2 ? cell_bucket<2>(cd, p)
+                                                                                 : cell_bucket<3>(cd, p);
+        const uint64_t e = ((uint64_t)cid << 32) | (uint32_t)lin;
+        uint64_t h = lintab_hash(lin) & (cap - 1);
+        while (atomicCAS((unsigned long long *)&tab[h], ~0ull, (unsigned long long)e) != ~0ull)
+            h = (h + 1) & (cap - 1);
+        ++cid;
+    }
 }
 
-__global__ void k_cloud_cells(CloudDev *__restrict__ desc, int32_t B, const int64_t *__restrict__ r_start,
-                              const uint32_t *__restrict__ flags, const uint32_t *__restrict__ cellpos) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= B) return;
-    const int64_t nr = r_start[c + 1] - r_start[c];
-    if (nr > 0) {
-        const int64_t last = r_start[c + 1] - 1;
-        desc[c].cell_off = cellpos[r_start[c]];
-        desc[c].ncells = (int64_t)cellpos[last] + flags[last] - cellpos[r_start[c]];
-    } else {
-        desc[c].cell_off = 0;
-        desc[c].ncells = 0;
+// Per-cloud cell counts from the first-cell numbers.
+__global__ void k_cloud_cells(CloudDev *__restrict__ desc, int32_t B, const uint32_t *__restrict__ misc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t next = misc[kMiscCells];
+    for (int c = B - 1; c >= 0; --c) {
+        if (desc[c].h.nr == 0 && !(desc[c].h.self_search && desc[c].h.nq > 0)) {
+            desc[c].cell_off = next;
+            desc[c].ncells = 0;
+            continue;
+        }
+        desc[c].ncells = next - desc[c].cell_off;
+        next = desc[c].cell_off;
     }
 }
+
+static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)((n + t - 1) / t); }
+
+int build_structure(const Batch &b, Work &w, hipStream_t s) {
+    const int32_t B = b.B;
+    int rc = fill_u32(w.misc, 0u, kMiscWords, s);
+    if (rc) return rc;
+    // hMax + grid: one launch
+    bool ref_side = false;
+    for (const CloudDev &d : b.dev)
+        if (!d.h.self_search && d.h.mode != TCR_MODE_GATHER && d.h.nr > 0) ref_side = true;
+    const int nbq = (int)std::min<int64_t>(512, blocks_for(b.nq_total));
+    const int nbr = ref_side ? (int)std::min<int64_t>(512, blocks_for(b.nr_total)) : 0;
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, B,
+                       (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
+                       w.misc);
+    TCR_LAUNCH_CHECK();
+    const int64_t n = b.nr_total;
+    if (n == 0) return TCR_OK;
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.r_start, n,
+                       w.rwrap, w.keys0, w.vals0, w.pflags);
+    TCR_LAUNCH_CHECK();
+    rc = radix_sort(b, w, s);
+    if (rc) return rc;
+    rc = fill_u32(w.lintab, 0xffffffffu, 2 * (size_t)b.lt_total, s);
+    if (rc) return rc;
+    rc = fill_u32(w.cell_lb, 0u, 2 * (size_t)b.rt_total, s);
+    if (rc) return rc;
+    CellsArgs ca;
+    ca.desc = w.desc;
+    ca.B = B;
+    ca.ntiles = b.rt_total;
+    ca.n = n;
+    ca.t_start = w.rt_start;
+    ca.p_start = w.r_start;
+    ca.skey = w.skey;
+    ca.sorted = w.sorted;
+    ca.cell_begin = w.cell_begin;
+    ca.cell_lin = w.cell_lin;
+    ca.cell_hash = w.cell_hash;
+    ca.cell_cloud = w.cell_cloud;
+    ca.lintab = w.lintab;
+    ca.lb = w.cell_lb;
+    ca.misc = w.misc;
+    hipLaunchKernelGGL(k_cells, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, ca);
+    hipLaunchKernelGGL(k_cloud_cells, dim3(1), dim3(64), 0, s, w.desc, B, (const uint32_t *)w.misc);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
+
+// ------------------------------------------ reference-layout bucket tables --
 
 __global__ void __launch_bounds__(kBlock) k_bucket_count(const CloudDev *__restrict__ desc, int64_t ncap,
                                                          const uint32_t *__restrict__ misc,
@@ -250,7 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_count(const CloudDev *__restr
                                                          const uint32_t *__restrict__ cell_cloud,
                                                          uint32_t *__restrict__ bucket_cnt) {
     const int64_t cid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cid >= ncap || cid >= (int64_t)misc[0]) return;
+    if (cid >= ncap || cid >= (int64_t)misc[kMiscCells]) return;
     const CloudDev &cd = desc[cell_cloud[cid]];
     atomicAdd(&bucket_cnt[cd.hb_off + cell_hash[cid]], 1u);
 }
@@ -269,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_scatter(const CloudDev *__res
                                                            uint32_t *__restrict__ bucket_fill,
                                                            int32_t *__restrict__ cell_table) {
     const int64_t cid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cid >= ncap || cid >= (int64_t)misc[0]) return;
+    if (cid >= ncap || cid >= (int64_t)misc[kMiscCells]) return;
     const uint32_t c = cell_cloud[cid];
     const CloudDev &cd = desc[c];
     const int64_t b = cd.hb_off + cell_hash[cid];
@@ -292,56 +458,25 @@ __global__ void __launch_bounds__(kBlock) k_hash_table(const CloudDev *__restric
     hash_table[2 * b + 1] = (int32_t)cnt;
 }
 
-static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)((n + t - 1) / t); }
-
-int build_structure(const Batch &b, Work &w, hipStream_t s) {
-    const int32_t B = b.B;
-    if (b.nq_total > 0)
-        hipLaunchKernelGGL(k_hmax, dim3((unsigned)std::min<int64_t>(kHmaxBlocks, blocks_for(b.nq_total))),
-                           dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.q_start, b.nq_total, 0);
-    if (b.nr_total > 0)
-        hipLaunchKernelGGL(k_hmax, dim3((unsigned)std::min<int64_t>(kHmaxBlocks, blocks_for(b.nr_total))),
-                           dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.r_start, b.nr_total, 1);
-    hipLaunchKernelGGL(k_grid, dim3(blocks_for(B, 64)), dim3(64), 0, s, w.desc, B);
-    TCR_LAUNCH_CHECK();
+int build_bucket_tables(const Batch &b, Work &w, hipStream_t s) {
     const int64_t n = b.nr_total;
-    if (n > 0) {
-        hipLaunchKernelGGL(k_ref_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
-                           (const int64_t *)w.r_start, n, w.rwrap, w.keys0, w.vals0, w.pflags);
-        TCR_LAUNCH_CHECK();
-        int rc = radix_sort_ref(b, w, 4, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_gather_sorted, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
-                           (const int64_t *)w.r_start, n, (const uint32_t *)w.keys0, (const uint32_t *)w.vals0,
-                           (const float4 *)w.rwrap, w.sorted, w.sort_index, w.skey, w.flags,
-                           (const uint8_t *)w.pflags, w.sflags);
-        rc = exclusive_scan<uint32_t, uint32_t>(w.flags, w.cellpos, n, false, w.scan_part, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_cell_emit, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
-                           (const int64_t *)w.r_start, n, (const uint32_t *)w.flags, (const uint32_t *)w.cellpos,
-                           (const int32_t *)w.skey, (const float4 *)w.sorted, w.cell_begin, w.cell_lin,
-                           w.cell_hash, w.cell_cloud, w.misc);
-        TCR_LAUNCH_CHECK();
-    } else {
-        TCR_HIP(hipMemsetAsync(w.misc, 0, sizeof(uint32_t), s));
-    }
-    hipLaunchKernelGGL(k_cloud_cells, dim3(blocks_for(B, 64)), dim3(64), 0, s, w.desc, B,
-                       (const int64_t *)w.r_start, (const uint32_t *)w.flags, (const uint32_t *)w.cellpos);
-    TCR_HIP(hipMemsetAsync(w.bucket_cnt, 0, sizeof(uint32_t) * (size_t)b.hb_total, s));
-    TCR_HIP(hipMemsetAsync(w.bucket_fill, 0, sizeof(uint32_t) * (size_t)b.hb_total, s));
+    int rc = fill_u32(w.bucket_cnt, 0u, (size_t)b.hb_total, s);
+    if (rc) return rc;
+    rc = fill_u32(w.bucket_fill, 0u, (size_t)b.hb_total, s);
+    if (rc) return rc;
     if (n > 0)
         hipLaunchKernelGGL(k_bucket_count, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, n,
                            (const uint32_t *)w.misc, (const uint32_t *)w.cell_hash, (const uint32_t *)w.cell_cloud,
                            w.bucket_cnt);
-    int rc = exclusive_scan<uint32_t, uint32_t>(w.bucket_cnt, w.bucket_begin, b.hb_total, false, w.scan_part, s);
+    rc = exclusive_scan<uint32_t, uint32_t>(w.bucket_cnt, w.bucket_begin, b.hb_total, false, w.scan_part, s);
     if (rc) return rc;
     if (n > 0)
         hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, n,
                            (const uint32_t *)w.misc, (const uint32_t *)w.cell_hash, (const uint32_t *)w.cell_cloud,
                            (const uint32_t *)w.cell_begin, (const int32_t *)w.cell_lin, (const int64_t *)w.r_start,
                            (const uint32_t *)w.bucket_begin, w.bucket_fill, w.cell_table);
-    hipLaunchKernelGGL(k_hash_table, dim3(blocks_for(b.hb_total)), dim3(kBlock), 0, s, (const CloudDev *)w.desc, B,
-                       (const int64_t *)w.hb_start, b.hb_total, (const uint32_t *)w.bucket_cnt,
+    hipLaunchKernelGGL(k_hash_table, dim3(blocks_for(b.hb_total)), dim3(kBlock), 0, s, (const CloudDev *)w.desc,
+                       b.B, (const int64_t *)w.hb_start, b.hb_total, (const uint32_t *)w.bucket_cnt,
                        (const uint32_t *)w.bucket_begin, w.hash_table);
     TCR_LAUNCH_CHECK();
     return TCR_OK;

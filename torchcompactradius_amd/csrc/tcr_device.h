@@ -30,6 +30,9 @@ struct CloudDev {
     int32_t nc[3];
     uint32_t err;             // nonzero: degenerate grid
     uint32_t fast;            // nonzero: wavefront-per-cell path eligible
+    uint32_t irregular;       // nonzero: some reference point's cell is outside the grid
+    uint32_t pad0;
+    int64_t lt_off, lt_cap;   // lin -> cell table slots of this cloud (cap: power of two)
     int64_t cell_off, ncells; // cells of this cloud in the batch cell arrays
     int64_t qcell_off, nqcells; // query groups (non-self) in the batch
 };
@@ -62,7 +65,9 @@ __device__ __forceinline__ int find_segment(int n, T i, F starts) {
 // wrap p' = remainder(p - min, max - min) + min (py/util.py:387,
 // py/compactHashing/cppSearch.py:221).  exact.
 __device__ __forceinline__ float wrap_coord(float p, float lo, float ext) {
-    float r = fmodf(p - lo, ext);
+    const float a = p - lo;
+    // fmod(a, b) == a exactly whenever |a| < |b| (NaN and b == 0 fail the test)
+    float r = fabsf(a) < fabsf(ext) ? a : fmodf(a, ext);
     if (r != 0.0f && ((r < 0.0f) != (ext < 0.0f))) r += ext;
     return r + lo;
 }

@@ -117,11 +117,11 @@ struct GroupArgs {
     const CloudDev *desc;
     const int64_t *r_start;
     const int64_t *q_start;
-    const uint32_t *misc;        // [0] occupied cells, [1] strays, [2] big cells
+    const uint32_t *misc;        // device counters (kMisc* slots)
     const uint32_t *cell_begin;  // [cells+1] first global sorted slot
     const uint32_t *cell_cloud;
-    const int32_t *hash_table;
-    const int32_t *cell_table;
+    const uint32_t *cell_hash;   // [cells] bucket of each cell
+    const uint64_t *lintab;      // lin -> cell tables of all clouds
     const float4 *sorted;
     const int32_t *sort_index;   // local original index per sorted slot
     const uint8_t *sflags;       // per sorted slot: nonzero = stray query
@@ -157,9 +157,10 @@ __device__ __forceinline__ bool cell_coords(const CloudDev &cd, float4 p0, int32
     return inrange;
 }
 
+// Stencil cell o of the cell at q: (first global sorted slot, count), or
+// (0, 0) when the bucket lookup of the reference finds no such cell.
 template <int DIM>
-__device__ __forceinline__ int2 stencil_one(const GroupArgs &a, const CloudDev &cd, const int32_t *q, int o,
-                                            int64_t rs) {
+__device__ __forceinline__ int2 stencil_one(const GroupArgs &a, const CloudDev &cd, const int32_t *q, int o) {
     int32_t cc[3];
     int32_t rem = o;
 #pragma unroll
@@ -170,18 +171,14 @@ __device__ __forceinline__ int2 stencil_one(const GroupArgs &a, const CloudDev &
     }
     const int32_t lin = linear_index<DIM>(cc, cd.nc);
     const uint32_t hb = hash_index<DIM>(cc, cd.h.hash_map_length);
-    const int32_t *ht = a.hash_table + 2 * cd.hb_off;
-    const int32_t *ct = a.cell_table + 3 * cd.cell_off;
-    const int32_t b0 = ht[2 * (int64_t)hb], bl = ht[2 * (int64_t)hb + 1];
-    if (b0 != -1)
-        for (int32_t i = b0; i < b0 + bl; ++i)
-            if (ct[3 * (int64_t)i] == lin)
-                return make_int2((int32_t)(rs + ct[3 * (int64_t)i + 1]), ct[3 * (int64_t)i + 2]);
-    return make_int2(0, 0);
+    const int32_t cid = lintab_find(a.lintab + cd.lt_off, (uint64_t)cd.lt_cap, lin);
+    if (cid < 0 || a.cell_hash[cid] != hb) return make_int2(0, 0);
+    const uint32_t b = a.cell_begin[cid];
+    return make_int2((int32_t)b, (int32_t)(a.cell_begin[cid + 1] - b));
 }
 
 __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
-    const uint32_t ncells = a.misc[0];
+    const uint32_t ncells = a.misc[kMiscCells];
     const int64_t n = (int64_t)ncells * kStencil;
     for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
         const uint32_t cid = (uint32_t)(g / kStencil);
@@ -207,11 +204,9 @@ __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
                                   make_float4(cen[0], cen[1], cen[2], 0.0f)};
         }
         int2 r = make_int2(0, 0);
-        if (fast && o < noff) {
-            const int64_t rs = a.r_start[c];
-            r = dim == 1 ? stencil_one<1>(a, cd, q, o, rs) : dim == 2 ? stencil_one<2>(a, cd, q, o, rs)
-                                                                      : stencil_one<3>(a, cd, q, o, rs);
-        }
+        if (fast && o < noff)
+            r = dim == 1 ? stencil_one<1>(a, cd, q, o) : dim == 2 ? stencil_one<2>(a, cd, q, o)
+                                                                  : stencil_one<3>(a, cd, q, o);
         a.ranges[g] = r;
     }
 }
@@ -664,7 +659,7 @@ template <int DIM, int MODE>
 __global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
     constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
-    const CellRange cr = my_cells(a.misc[0]);
+    const CellRange cr = my_cells(a.misc[kMiscCells]);
     if (cr.first >= cr.end) return;
     CellLoad cur = load_cell(a, cr.first);
     for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
@@ -688,7 +683,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
 __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
-    const CellRange cr = my_cells(a.misc[0]);
+    const CellRange cr = my_cells(a.misc[kMiscCells]);
     if (cr.first >= cr.end) return;
     CellLoad cur = load_cell(a, cr.first);
     for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
@@ -778,8 +773,8 @@ static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64
     ga.misc = w.misc;
     ga.cell_begin = w.cell_begin;
     ga.cell_cloud = w.cell_cloud;
-    ga.hash_table = w.hash_table;
-    ga.cell_table = w.cell_table;
+    ga.cell_hash = w.cell_hash;
+    ga.lintab = w.lintab;
     ga.sorted = w.sorted;
     ga.sort_index = w.sort_index;
     ga.sflags = w.sflags;
@@ -793,9 +788,9 @@ static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64
     ga.col = col;
     ga.cursor = w.cursor;
     ga.stray_list = w.stray_list;
-    ga.stray_count = w.misc + 1;
+    ga.stray_count = w.misc + kMiscStrays;
     ga.big_list = w.big_list;
-    ga.big_count = w.misc + 2;
+    ga.big_count = w.misc + kMiscBig;
     return ga;
 }
 
@@ -814,7 +809,8 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
     if (any_group) {
         const GroupArgs ga = group_args(w, offsets, row, col);
         if (!fill) {
-            TCR_HIP(hipMemsetAsync(w.misc + 1, 0, 2 * sizeof(uint32_t), s));
+            // misc[kMiscStrays] and misc[kMiscBig] start at zero: the
+            // build clears all of misc
             hipLaunchKernelGGL(k_stencil, dim3(2048), dim3(256), 0, s, ga);
         } else {
             hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((b.nr_total + 255) / 256)), dim3(256), 0, s,

@@ -15,10 +15,23 @@ constexpr int kSortItems = 16;             // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 4;              // 32-bit keys
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+// misc[] slots (device counters of one search)
+enum : int {
+    kMiscCells = 0,      // occupied cells of the batch
+    kMiscStrays = 1,     // stray queries listed by the count pass
+    kMiscBig = 2,        // cells with more candidates than one register chunk
+    kMiscPasses = 3,     // radix passes of this search
+    kMiscPrepTicket = 4, // last-block ticket of k_prep
+    kMiscCellTicket = 5, // tile ticket of k_cells
+    kMiscSortTicket = 8, // tile tickets of the radix passes (kMaxPasses)
+    kMiscWords = 16,
+};
 
 // Error reporting (thread-local message; see tcr_last_error()).
 void set_error(const std::string &msg);
@@ -58,13 +71,11 @@ struct Carve {
 struct Batch {
     int32_t B = 0;
     int64_t nq_total = 0, nr_total = 0;   // points
-    int64_t rt_total = 0, qt_total = 0;   // sort tiles
-    int64_t hb_total = 0;                 // hash buckets
-    int64_t qf_total = 0, rf_total = 0;   // floats
+    int64_t rt_total = 0;                 // sort tiles (reference points)
+    int64_t hb_total = 0;                 // hash buckets (reference layout)
+    int64_t lt_total = 0;                 // lin-table slots
     bool any_nonself = false;
-    int max_dim = 1;
     std::vector<CloudDev> dev;            // host copy of the descriptors
-    std::vector<int64_t> rt_prefix;       // [B+1]
 };
 
 // Workspace pointers.
@@ -73,31 +84,32 @@ struct Work {
     int64_t *q_start;     // [B+1] query point prefix
     int64_t *r_start;     // [B+1] reference point prefix
     int64_t *rt_start;    // [B+1] reference tile prefix
-    int64_t *qt_start;    // [B+1] query tile prefix
-    int64_t *hb_start;    // [B+1] bucket prefix
+    int64_t *hb_start;    // [B+1] bucket prefix (reference-layout tables)
+    int64_t *lt_start;    // [B+1] lin-table slot prefix (power-of-two per cloud)
+    uint32_t *misc;       // [kMiscWords]
     float4 *rwrap;        // [Nr] wrapped reference positions (+support in .w)
-    uint32_t *keys0, *keys1;
-    uint32_t *vals0, *vals1;
-    uint32_t *hist;       // [kRadix * tiles]
-    uint64_t *scan_part;  // scan partials
+    uint32_t *keys0, *keys1;  // cell linear index (raw int32 bits)
+    uint32_t *vals0, *vals1;  // local original index
+    uint8_t *pflags;      // [Nr] stray bits per reference point (input order)
+    uint32_t *sort_hist;  // [B * kMaxPasses * kRadix] digit counts per cloud
+    uint64_t *sort_lb;    // [kMaxPasses * tiles * kRadix] look-back state
+    uint64_t *cell_lb;    // [tiles] look-back state of the cell extraction
     float4 *sorted;       // [Nr] sorted positions (+support)
     int32_t *sort_index;  // [Nr] sorted slot -> local original index
     int32_t *skey;        // [Nr] sorted linear cell index
-    uint32_t *flags;      // [Nr]
-    uint32_t *cellpos;    // [Nr]
+    uint8_t *sflags;      // [Nr] stray bits per sorted slot
     uint32_t *cell_begin; // [Nr+1] first global slot of each cell
     int32_t *cell_lin;    // [Nr]
-    uint32_t *cell_hash;  // [Nr]
+    uint32_t *cell_hash;  // [Nr] bucket of the cell (hash of its first point's cell)
     uint32_t *cell_cloud; // [Nr]
-    uint32_t *bucket_cnt; // [HB]
+    uint64_t *lintab;     // [LT] (cell id << 32 | lin), all ones = empty
+    uint32_t *bucket_cnt; // [HB] (export)
     uint32_t *bucket_begin;
     uint32_t *bucket_fill;
-    int32_t *cell_table;  // [Nr*3] reference layout, grouped by bucket
-    int32_t *hash_table;  // [HB*2]
+    int32_t *cell_table;  // [Nr*3] reference layout, grouped by bucket (export)
+    int32_t *hash_table;  // [HB*2] (export)
+    uint64_t *scan_part;  // scan partials
     int32_t *counts;      // [Nq]
-    uint32_t *misc;       // small counters: [0] cells, [1] strays
-    uint8_t *pflags;      // [Nr] stray bits per reference point (input order)
-    uint8_t *sflags;      // [Nr] stray bits per sorted slot
     int32_t *cursor;      // [Nq] per-query write cursor across chunks
     int32_t *stray_list;  // [Nq] global query indices for the per-query path
     uint64_t *masks;      // [Nr * 7] per sorted slot: hit mask of each candidate round
@@ -107,7 +119,8 @@ struct Work {
     uint32_t *big_list;   // [Nr] cells with more candidates than one register chunk
 };
 
-// One cloud's grid and tables as the traversal reads them.
+// Bucket layout of the reference (hashTable [L,2] + cellTable [C,3]); used
+// by the reference-layout ops.
 struct GridView {
     const int32_t *hash_table;  // [L,2] of this cloud
     const int32_t *cell_table;  // [C,3] of this cloud
@@ -136,6 +149,9 @@ int hash_indices_launch(const int32_t *cells, int64_t n, int dim, uint32_t L, in
 bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err);
 size_t carve_work(const Batch &b, void *ws, Work &w);
 
+// Fills n 32-bit words with v (a kernel, so it is safe inside graph capture).
+int fill_u32(void *p, uint32_t v, size_t n, hipStream_t s);
+
 // Generic device-wide exclusive scan: out[i] = sum(in[0..i)), and
 // out[n] = total when write_total.  Tin: uint32_t or int32_t;
 // Tout: uint32_t or int64_t.
@@ -144,8 +160,27 @@ int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64
                    hipStream_t s);
 size_t scan_partials_needed(int64_t n);
 
-// Stable LSD radix sort of the reference keys of every cloud (segmented by
-// cloud); result in (keys0, vals0) after an even number of passes.
-int radix_sort_ref(const Batch &b, Work &w, int passes, hipStream_t s);
+// Structure build (hMax, grid, keys, sort, cells, lin table).
+int build_structure(const Batch &b, Work &w, hipStream_t s);
+// Reference-layout bucket tables of every cloud (for export).
+int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
+// Segmented stable radix sort of the reference keys with the sorted
+// positions gathered by its last pass.
+int radix_sort(const Batch &b, Work &w, hipStream_t s);
+
+// lin -> cell lookup with the reference's bucket semantics: the class with
+// linear index `lin` answers a lookup for cell coordinates whose bucket is
+// `bucket` only if the class lives in that bucket (its first point's cell
+// hashes there) -- the outcome of queryHashMap (cpp/neighborhood.h:35-58).
+__device__ __forceinline__ uint32_t lintab_hash(int32_t lin) { return (uint32_t)lin * 0x9E3779B1u; }
+__device__ __forceinline__ int32_t lintab_find(const uint64_t *__restrict__ tab, uint64_t cap, int32_t lin) {
+    uint64_t h = lintab_hash(lin) & (cap - 1);
+    while (true) {
+        const uint64_t e = tab[h];
+        if (e == ~0ull) return -1;
+        if ((int32_t)(uint32_t)e == lin) return (int32_t)(e >> 32);
+        h = (h + 1) & (cap - 1);
+    }
+}
 
 }  // namespace tcr

@@ -5,9 +5,10 @@
 // queryHashMap (cpp/neighborhood.h:35-94): query cell from the once-wrapped
 // query position, (2r+1)^d stencil in lexicographic order (dim 0 slowest),
 // periodic neighbour cells through pymod, non-periodic ones unclamped, the
-// bucket scan matching only the linear index, the exact modDistance and the
-// strict support predicate.  One lane per query; this is the general path
-// that every input shape can take.
+// bucket lookup matching only the linear index, the exact modDistance and
+// the strict support predicate.  One lane per query; this is the general
+// path that every input shape can take (non-self searches, stencil radius
+// > 1, and the strays of the wavefront-per-cell path).
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -33,11 +34,54 @@ struct RowCands {
     }
 };
 
-// Visits the hits of query xi in reference order; emit(k, j_sorted) for the
-// k-th hit.  Returns the hit count.
-template <int DIM, class Cands, class Emit>
-__device__ __forceinline__ int64_t traverse(const GridView &g, const Cands &cands, const float *xi, float hi,
-                                            Emit emit) {
+// Reference layout: bucket hb of hashTable [L,2], scanned for the linear
+// index (queryHashMap, cpp/neighborhood.h:35-58).  Returns [begin, end)
+// relative to the candidate storage.
+struct BucketLookup {
+    const int32_t *hash_table, *cell_table;
+    __device__ __forceinline__ bool find(int32_t lin, uint32_t hb, int64_t &b, int64_t &e) const {
+        const int32_t b0 = hash_table[2 * (int64_t)hb], bl = hash_table[2 * (int64_t)hb + 1];
+        if (b0 == -1) return false;
+        for (int32_t i = b0; i < b0 + bl; ++i)
+            if (cell_table[3 * (int64_t)i] == lin) {
+                b = cell_table[3 * (int64_t)i + 1];
+                e = b + cell_table[3 * (int64_t)i + 2];
+                return true;
+            }
+        return false;
+    }
+};
+
+// Search structure of this library: the cell with linear index lin, if it
+// exists and lives in bucket hb -- the same answer as the bucket scan.
+// Returns global sorted slots minus the cloud's first slot.
+struct LinLookup {
+    const uint64_t *tab;
+    uint64_t cap;
+    const uint32_t *cell_hash, *cell_begin;
+    int64_t rs;
+    __device__ __forceinline__ bool find(int32_t lin, uint32_t hb, int64_t &b, int64_t &e) const {
+        const int32_t cid = lintab_find(tab, cap, lin);
+        if (cid < 0 || cell_hash[cid] != hb) return false;
+        b = (int64_t)cell_begin[cid] - rs;
+        e = (int64_t)cell_begin[cid + 1] - rs;
+        return true;
+    }
+};
+
+struct Geom {
+    int32_t nc[3];
+    uint32_t pmask, L;
+    float mn[3], mx[3];
+    float hcell;
+    int32_t sr, mode;
+};
+
+// Visits the hits of query xi in reference order; emit(k, j) for the k-th
+// hit (j relative to the candidate storage).  Returns the hit count.
+template <int DIM, class Lookup, class Cands, class Emit>
+__device__ __forceinline__ int64_t traverse(const Geom &g, const Lookup &lk, const Cands &cands, const float *xi,
+                                            float hi, Emit emit) {
     int32_t q[3];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) q[d] = cell_coord(xi[d], g.mn[d], g.hcell);
@@ -55,19 +99,9 @@ __device__ __forceinline__ int64_t traverse(const GridView &g, const Cands &cand
             rem /= w;
             c[d] = ((g.pmask >> d) & 1u) ? pymod(q[d] + off, g.nc[d]) : q[d] + off;
         }
-        const int32_t lin = linear_index<DIM>(c, g.nc);
-        const uint32_t hb = hash_index<DIM>(c, g.L);
-        const int32_t b0 = g.hash_table[2 * (int64_t)hb], bl = g.hash_table[2 * (int64_t)hb + 1];
-        if (b0 == -1) continue;
-        int32_t cb = -1, ce = -1;
-        for (int32_t i = b0; i < b0 + bl; ++i) {
-            if (g.cell_table[3 * (int64_t)i] == lin) {
-                cb = g.cell_table[3 * (int64_t)i + 1];
-                ce = cb + g.cell_table[3 * (int64_t)i + 2];
-                break;
-            }
-        }
-        for (int32_t j = cb; j < ce; ++j) {
+        int64_t cb, ce;
+        if (!lk.find(linear_index<DIM>(c, g.nc), hash_index<DIM>(c, g.L), cb, ce)) continue;
+        for (int64_t j = cb; j < ce; ++j) {
             float xj[3], hj;
             cands.get(j, xj, hj);
             const float dist = exact_distance<DIM>(xi, xj, g.mn, g.mx, g.pmask);
@@ -80,18 +114,15 @@ __device__ __forceinline__ int64_t traverse(const GridView &g, const Cands &cand
     return k;
 }
 
-__device__ __forceinline__ GridView cloud_view(const CloudDev &cd, const int32_t *hash_table,
-                                               const int32_t *cell_table) {
-    GridView g;
-    g.hash_table = hash_table + 2 * cd.hb_off;
-    g.cell_table = cell_table + 3 * cd.cell_off;
-    g.L = cd.h.hash_map_length;
+__device__ __forceinline__ Geom cloud_geom(const CloudDev &cd) {
+    Geom g;
     for (int d = 0; d < 3; ++d) {
         g.nc[d] = cd.nc[d];
         g.mn[d] = cd.h.domain_min[d];
         g.mx[d] = cd.h.domain_max[d];
     }
     g.pmask = cd.h.periodic_mask;
+    g.L = cd.h.hash_map_length;
     g.hcell = cd.hcell;
     g.sr = cd.h.search_range;
     g.mode = cd.h.mode;
@@ -112,41 +143,13 @@ __device__ __forceinline__ void load_query(const CloudDev &cd, int64_t local, fl
     hi = cd.qs ? cd.qs[local] : 0.0f;
 }
 
-template <int DIM, bool FILL>
-__device__ __forceinline__ void query_one(const CloudDev &cd, int64_t g, int64_t local,
-                                          const int32_t *hash_table, const int32_t *cell_table,
-                                          const float4 *sorted, const int64_t *r_start_c,
-                                          const int32_t *sort_index, int32_t *counts,
-                                          const int64_t *offsets, int64_t *row, int64_t *col) {
-    float xi[3], hi;
-    load_query<DIM>(cd, local, xi, hi);
-    const GridView gv = cloud_view(cd, hash_table, cell_table);
-    const int64_t rs = *r_start_c;
-    Float4Cands cands{sorted + rs};
-    if (!FILL) {
-        counts[g] = (int32_t)traverse<DIM>(gv, cands, xi, hi, [](int64_t, int32_t) {});
-    } else {
-        const int64_t base = offsets[g];
-        const int32_t *si = sort_index + rs;
-        traverse<DIM>(gv, cands, xi, hi, [&](int64_t k, int32_t j) {
-            col[base + k] = (int64_t)si[j];
-            if (row) row[base + k] = local;
-        });
-    }
-}
-
-// Clouds whose queries the wavefront-per-cell kernels own (their strays
-// come back through the stray list).
-__device__ __forceinline__ bool group_owned(const CloudDev &cd) {
-    return cd.h.self_search && cd.h.search_range == 1;
-}
-
 struct GenericArgs {
     const CloudDev *desc;
     int32_t B;
     const int64_t *q_start, *r_start;
     int64_t nq;
-    const int32_t *hash_table, *cell_table;
+    const uint64_t *lintab;
+    const uint32_t *cell_hash, *cell_begin;
     const float4 *sorted;
     const int32_t *sort_index;
     int32_t *counts;
@@ -156,16 +159,43 @@ struct GenericArgs {
     const uint32_t *list_count;
 };
 
+template <int DIM, bool FILL>
+__device__ __forceinline__ void query_one(const GenericArgs &a, const CloudDev &cd, int c, int64_t g,
+                                          int64_t local) {
+    float xi[3], hi;
+    load_query<DIM>(cd, local, xi, hi);
+    const Geom gm = cloud_geom(cd);
+    const int64_t rs = a.r_start[c];
+    const LinLookup lk{a.lintab + cd.lt_off, (uint64_t)cd.lt_cap, a.cell_hash, a.cell_begin, rs};
+    const Float4Cands cands{a.sorted + rs};
+    if (!FILL) {
+        a.counts[g] = (int32_t)traverse<DIM>(gm, lk, cands, xi, hi, [](int64_t, int64_t) {});
+    } else {
+        const int64_t base = a.offsets[g];
+        const int32_t *si = a.sort_index + rs;
+        traverse<DIM>(gm, lk, cands, xi, hi, [&](int64_t k, int64_t j) {
+            a.col[base + k] = (int64_t)si[j];
+            if (a.row) a.row[base + k] = local;
+        });
+    }
+}
+
 template <bool FILL>
 __device__ __forceinline__ void generic_query(const GenericArgs &a, int64_t g) {
     const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
     const CloudDev &cd = a.desc[c];
     const int64_t local = g - a.q_start[c];
     switch (cd.h.dim) {
-        case 1: query_one<1, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
-        case 2: query_one<2, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
-        default: query_one<3, FILL>(cd, g, local, a.hash_table, a.cell_table, a.sorted, a.r_start + c, a.sort_index, a.counts, a.offsets, a.row, a.col); break;
+        case 1: query_one<1, FILL>(a, cd, c, g, local); break;
+        case 2: query_one<2, FILL>(a, cd, c, g, local); break;
+        default: query_one<3, FILL>(a, cd, c, g, local); break;
     }
+}
+
+// Clouds whose queries the wavefront-per-cell kernels own (their strays
+// come back through the stray list).
+__device__ __forceinline__ bool owned_by_cells(const CloudDev &cd) {
+    return cd.h.self_search && cd.h.search_range == 1;
 }
 
 // Every query of the clouds the wavefront-per-cell kernels do not own.
@@ -174,7 +204,7 @@ __global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
     const int64_t g = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
     if (g >= a.nq) return;
     const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
-    if (group_owned(a.desc[c])) return;
+    if (owned_by_cells(a.desc[c])) return;
     generic_query<FILL>(a, g);
 }
 
@@ -186,15 +216,17 @@ __global__ void __launch_bounds__(kQBlock) k_query_list(GenericArgs a) {
         generic_query<FILL>(a, a.list[i]);
 }
 
-static GenericArgs generic_args(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col) {
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
+                   bool list, hipStream_t s) {
     GenericArgs a;
     a.desc = w.desc;
     a.B = b.B;
     a.q_start = w.q_start;
     a.r_start = w.r_start;
     a.nq = b.nq_total;
-    a.hash_table = w.hash_table;
-    a.cell_table = w.cell_table;
+    a.lintab = w.lintab;
+    a.cell_hash = w.cell_hash;
+    a.cell_begin = w.cell_begin;
     a.sorted = w.sorted;
     a.sort_index = w.sort_index;
     a.counts = w.counts;
@@ -202,15 +234,9 @@ static GenericArgs generic_args(const Batch &b, Work &w, const int64_t *offsets,
     a.row = row;
     a.col = col;
     a.list = w.stray_list;
-    a.list_count = w.misc + 1;
-    return a;
-}
-
-int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
-                   bool list, hipStream_t s) {
-    const GenericArgs a = generic_args(b, w, offsets, row, col);
+    a.list_count = w.misc + kMiscStrays;
     if (list) {
-        const dim3 grid(1024), block(kQBlock);
+        const dim3 grid(256), block(kQBlock);
         if (fill) hipLaunchKernelGGL(k_query_list<true>, grid, block, 0, s, a);
         else hipLaunchKernelGGL(k_query_list<false>, grid, block, 0, s, a);
     } else {
@@ -233,12 +259,24 @@ __global__ void __launch_bounds__(kQBlock) k_refop(RefOpArgs a) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) xi[d] = a.qpos[i * DIM + d];
     const float hi = a.qsup ? a.qsup[i] : 0.0f;
-    RowCands<DIM> cands{a.spos, a.ssup};
+    Geom g;
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = a.g.nc[d];
+        g.mn[d] = a.g.mn[d];
+        g.mx[d] = a.g.mx[d];
+    }
+    g.pmask = a.g.pmask;
+    g.L = a.g.L;
+    g.hcell = a.g.hcell;
+    g.sr = a.g.sr;
+    g.mode = a.g.mode;
+    const BucketLookup lk{a.g.hash_table, a.g.cell_table};
+    const RowCands<DIM> cands{a.spos, a.ssup};
     if (!FILL) {
-        a.counts[i] = (int32_t)traverse<DIM>(a.g, cands, xi, hi, [](int64_t, int32_t) {});
+        a.counts[i] = (int32_t)traverse<DIM>(g, lk, cands, xi, hi, [](int64_t, int64_t) {});
     } else {
         const int64_t base = a.offsets32[i];
-        traverse<DIM>(a.g, cands, xi, hi, [&](int64_t k, int32_t j) {
+        traverse<DIM>(g, lk, cands, xi, hi, [&](int64_t k, int64_t j) {
             a.li[base + k] = i;
             a.lj[base + k] = j;
         });

@@ -1,80 +1,145 @@
 // Stable LSD radix sort of (cell key, point index) pairs into cell order, one
-// segment per cloud, wave64 multi-split ranking in LDS.  Replaces
-// torch.argsort of the int32 cell keys (py/compactHashing/cellTable.py:33)
-// with its stable contract: inside a cell, ascending original index.
+// segment per cloud -- the contract of torch.argsort on the int32 cell keys
+// (py/compactHashing/cellTable.py:33) on the reference's GPU path: inside a
+// cell, ascending original index.
 //
-// A tile is kSortTile keys of one cloud (tiles never straddle clouds).  The
-// per-tile digit histograms are laid out cloud-major, then digit, then tile,
-// so one exclusive scan over the whole histogram array yields the global
-// destination of every (cloud, digit, tile) run: the batch stays segmented by
-// cloud without any per-cloud launch.
+// Onesweep layout: one histogram kernel counts the digits of every pass for
+// every cloud, then one kernel per pass ranks a tile of 4096 keys in LDS
+// (wave64 multi-split with 8 ballots per round), finds the tile's global
+// offset per digit by decoupled look-back over the earlier tiles of its
+// cloud, and scatters.  Tiles are numbered by an atomic ticket, so a tile
+// only ever waits for tiles that are already running.  The number of passes
+// is decided on the device from the key range (regular clouds sort the
+// linear index in [0, prod numCells); a cloud with points outside the grid
+// or more than 2^31 cells sorts the sign-flipped 32-bit key), and the last
+// pass gathers the sorted
+// positions, supports, indices and stray bits directly.
 #include "tcr_internal.h"
 
 namespace tcr {
 
 struct TileInfo {
+    int c;           // cloud
     int64_t start;   // first global key of the tile
     int32_t len;     // keys in the tile
-    int64_t hbase;   // histogram base of the cloud (kRadix * first tile)
-    int64_t ntiles;  // tiles of the cloud
-    int64_t lt;      // tile index inside the cloud
+    int64_t first;   // first tile of the cloud
 };
 
 __device__ __forceinline__ TileInfo tile_info(int64_t t, int32_t B, const int64_t *__restrict__ t_start,
                                               const int64_t *__restrict__ p_start) {
-    // Empty clouds own zero tiles; find the last cloud whose first tile <= t
-    // and which owns tiles.
+    // last cloud whose first tile <= t (clouds without tiles share their
+    // successor's first tile and are skipped by taking the last match)
     const int c = find_segment(B, t, [&](int k) { return t_start[k]; });
-    // clouds with zero tiles share t_start with their successor: advance.
-    int cc = c;
     TileInfo ti;
-    const int64_t lt = t - t_start[cc];
-    const int64_t n = p_start[cc + 1] - p_start[cc];
-    ti.start = p_start[cc] + lt * kSortTile;
+    ti.c = c;
+    ti.first = t_start[c];
+    const int64_t lt = t - t_start[c];
+    const int64_t n = p_start[c + 1] - p_start[c];
+    ti.start = p_start[c] + lt * kSortTile;
     const int64_t rem = n - lt * kSortTile;
     ti.len = (int32_t)(rem < kSortTile ? rem : kSortTile);
-    ti.hbase = (int64_t)kRadix * t_start[cc];
-    ti.ntiles = t_start[cc + 1] - t_start[cc];
-    ti.lt = lt;
     return ti;
 }
 
-__global__ void __launch_bounds__(kSortThreads)
-k_radix_hist(const uint32_t *__restrict__ keys, int32_t B, const int64_t *__restrict__ t_start,
-             const int64_t *__restrict__ p_start, int shift, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[kRadix];
-    for (int k = threadIdx.x; k < kRadix; k += kSortThreads) h[k] = 0;
-    __syncthreads();
-    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
-    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
-        const uint32_t d = (keys[ti.start + k] >> shift) & (kRadix - 1);
-        atomicAdd(&h[d], 1u);
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
-        hist[ti.hbase + (int64_t)d * ti.ntiles + ti.lt] = h[d];
+// Sort key: the int32 linear index in signed order.  A cloud whose keys all
+// lie in [0, prod numCells) < 2^31 sorts them unsigned (same order, fewer
+// passes); any other ("wide") cloud flips the sign bit and takes all passes.
+__device__ __forceinline__ uint32_t eff_key(uint32_t lin, bool wide) { return wide ? (lin ^ 0x80000000u) : lin; }
+
+__device__ __forceinline__ uint64_t grid_cells(const CloudDev &cd) {
+    uint64_t P = 1;
+    for (int d = 0; d < cd.h.dim; ++d) P *= (uint64_t)(cd.nc[d] > 0 ? cd.nc[d] : 1);
+    return P;
 }
 
-// Ranks each key among the equal-digit keys before it in tile order and
-// scatters (key, val) to its global destination.  Wave w owns the contiguous
-// key range [w * 1024, (w + 1) * 1024) of the tile and walks it in 16 rounds
-// of 64 consecutive keys; peers with the same digit inside a round are found
-// with kRadixBits ballots.
+// Points outside the grid (INT32_MIN coordinates) or more than 2^31 cells
+// (linear indices wrap to negative values).
+__device__ __forceinline__ bool cloud_wide(const CloudDev &cd) {
+    return cd.irregular || grid_cells(cd) > (1ull << 31);
+}
+
+// Radix passes a cloud needs.
+__device__ __forceinline__ uint32_t cloud_passes(const CloudDev &cd) {
+    if (cloud_wide(cd)) return kMaxPasses;
+    const uint64_t P = grid_cells(cd);
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < P) ++bits;
+    return (bits + kRadixBits - 1) / kRadixBits;
+}
+
 __global__ void __launch_bounds__(kSortThreads)
-k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int32_t B,
-                const int64_t *__restrict__ t_start, const int64_t *__restrict__ p_start, int shift,
-                const uint32_t *__restrict__ hist_ex) {
+k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restrict__ t_start,
+            const int64_t *__restrict__ p_start, const uint32_t *__restrict__ keys, uint32_t *__restrict__ hist,
+            uint32_t *__restrict__ misc) {
+    __shared__ uint32_t h[kMaxPasses][kRadix];
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) (&h[0][0])[k] = 0;
+    __syncthreads();
+    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
+    const CloudDev &cd = desc[ti.c];
+    const bool irr = cloud_wide(cd);
+    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
+        const uint32_t ek = eff_key(keys[ti.start + k], irr);
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p) atomicAdd(&h[p][(ek >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) {
+        const uint32_t v = (&h[0][0])[k];
+        if (v) atomicAdd(&hc[k], v);
+    }
+    if (threadIdx.x == 0) atomicMax(&misc[kMiscPasses], cloud_passes(cd));
+}
+
+constexpr uint64_t kFlagAgg = 1ull << 32;
+constexpr uint64_t kFlagInc = 2ull << 32;
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct SortPassArgs {
+    const CloudDev *desc;
+    int32_t B;
+    int64_t ntiles;
+    const int64_t *t_start, *p_start;
+    const uint32_t *kin, *vin;
+    uint32_t *kout, *vout;
+    const uint32_t *hist;
+    uint64_t *lb;            // [ntiles * kRadix] of this pass
+    uint32_t *misc;
+    int pass;
+    // last pass: gather
+    const float4 *rwrap;
+    const uint8_t *pflags;
+    float4 *sorted;
+    int32_t *sort_index;
+    int32_t *skey;
+    uint8_t *sflags;
+};
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     constexpr int kWaves = kSortThreads / kWave;
     constexpr int kPerWave = kSortTile / kWaves;  // 1024
     __shared__ uint32_t wcnt[kWaves][kRadix];
-    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t dbase[kRadix];      // digit base of the cloud in this pass
+    __shared__ uint32_t tile_excl[kRadix];  // keys of this digit in earlier tiles of the cloud
+    __shared__ int64_t s_tile;
+    const uint32_t npass = a.misc[kMiscPasses];
+    if ((uint32_t)a.pass >= npass) return;
+    const bool last = (uint32_t)a.pass == npass - 1;
+    const int shift = a.pass * kRadixBits;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+
+    if (threadIdx.x == 0) s_tile = atomicAdd(&a.misc[kMiscSortTicket + a.pass], 1u);
     for (int k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&wcnt[0][0])[k] = 0;
-    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
-    for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
-        gbase[d] = hist_ex[ti.hbase + (int64_t)d * ti.ntiles + ti.lt];
     __syncthreads();
+    const int64_t tile = s_tile;
+    const TileInfo ti = tile_info(tile, a.B, a.t_start, a.p_start);
+    const bool irr = cloud_wide(a.desc[ti.c]);
 
     uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -82,9 +147,9 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
     for (int r = 0; r < kSortItems; ++r) {
         const int pos = wave * kPerWave + r * kWave + lane;
         const bool valid = pos < ti.len;
-        key[r] = valid ? kin[ti.start + pos] : 0u;
-        val[r] = valid ? vin[ti.start + pos] : 0u;
-        const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+        key[r] = valid ? a.kin[ti.start + pos] : 0u;
+        val[r] = valid ? a.vin[ti.start + pos] : 0u;
+        const uint32_t d = (eff_key(key[r], irr) >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
@@ -93,18 +158,17 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
             peers &= bit ? bal : ~bal;
         }
         const uint32_t before = wcnt[wave][d];
-        const uint32_t in_round = (uint32_t)__popcll(peers & lt_mask);
-        rank[r] = before + in_round;
-        // the lowest peer publishes the new running count of this digit
+        rank[r] = before + (uint32_t)__popcll(peers & lt_mask);
+        // the lowest peer publishes the new running count of this digit;
+        // LDS operations of one wavefront execute in program order, and the
+        // empty asm keeps the compiler from reordering them
         if (valid && (peers & lt_mask) == 0) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
-        // LDS operations of one wavefront execute in program order, so the
-        // next round's reads see this write; the empty asm keeps the
-        // compiler from moving them across each other.
         __asm__ volatile("" ::: "memory");
     }
     __syncthreads();
-    // exclusive prefix over waves per digit
-    for (int d = threadIdx.x; d < kRadix; d += kSortThreads) {
+    // thread d: tile count of digit d, wave prefixes, look-back, digit base
+    {
+        const int d = threadIdx.x;  // kSortThreads == kRadix
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
@@ -112,37 +176,94 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
             wcnt[w][d] = run;
             run += c;
         }
+        uint64_t *lb = a.lb + tile * kRadix + d;
+        if (tile == ti.first) {
+            lb_store(lb, kFlagInc | run);
+            tile_excl[d] = 0;
+        } else {
+            lb_store(lb, kFlagAgg | run);
+            uint64_t excl = 0;
+            int64_t t = tile - 1;
+            while (true) {
+                const uint64_t v = lb_load(a.lb + t * kRadix + d);
+                const uint64_t flag = v & ~0xffffffffull;
+                if (flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)v;
+                if (flag == kFlagInc) break;
+                --t;
+            }
+            lb_store(lb, kFlagInc | (excl + run));
+            tile_excl[d] = (uint32_t)excl;
+        }
+        dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + d];
     }
     __syncthreads();
+    // exclusive scan of the cloud's digit counts (one thread)
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kRadix; ++d) {
+            const uint32_t c = dbase[d];
+            dbase[d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const int64_t cbase = a.p_start[ti.c];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const int pos = wave * kPerWave + r * kWave + lane;
         if (pos < ti.len) {
-            const uint32_t d = (key[r] >> shift) & (kRadix - 1);
-            const uint32_t dst = gbase[d] + wcnt[wave][d] + rank[r];
-            kout[dst] = key[r];
-            vout[dst] = val[r];
+            const uint32_t d = (eff_key(key[r], irr) >> shift) & (kRadix - 1);
+            const int64_t dst = cbase + dbase[d] + tile_excl[d] + wcnt[wave][d] + rank[r];
+            if (!last) {
+                a.kout[dst] = key[r];
+                a.vout[dst] = val[r];
+            } else {
+                a.sorted[dst] = a.rwrap[cbase + val[r]];
+                a.sort_index[dst] = (int32_t)val[r];
+                a.skey[dst] = (int32_t)key[r];
+                a.sflags[dst] = a.pflags[cbase + val[r]];
+            }
         }
     }
 }
 
-int radix_sort_ref(const Batch &b, Work &w, int passes, hipStream_t s) {
+int radix_sort(const Batch &b, Work &w, hipStream_t s) {
     if (b.rt_total == 0) return TCR_OK;
-    const int64_t nh = (int64_t)kRadix * b.rt_total;
-    uint32_t *ki = w.keys0, *vi = w.vals0, *ko = w.keys1, *vo = w.vals1;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * kRadixBits;
-        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, ki, b.B,
-                           (const int64_t *)w.rt_start, (const int64_t *)w.r_start, shift, w.hist);
-        int rc = exclusive_scan<uint32_t, uint32_t>(w.hist, w.hist, nh, false, w.scan_part, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, ki, vi, ko,
-                           vo, b.B, (const int64_t *)w.rt_start, (const int64_t *)w.r_start, shift,
-                           (const uint32_t *)w.hist);
-        TCR_LAUNCH_CHECK();
-        uint32_t *t = ki; ki = ko; ko = t;
-        t = vi; vi = vo; vo = t;
+    int rc = fill_u32(w.sort_hist, 0u, (size_t)b.B * kMaxPasses * kRadix, s);
+    if (rc) return rc;
+    rc = fill_u32(w.sort_lb, 0u, 2 * (size_t)kMaxPasses * b.rt_total * kRadix, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, (const CloudDev *)w.desc,
+                       b.B, (const int64_t *)w.rt_start, (const int64_t *)w.r_start, (const uint32_t *)w.keys0,
+                       w.sort_hist, w.misc);
+    SortPassArgs a;
+    a.desc = w.desc;
+    a.B = b.B;
+    a.ntiles = b.rt_total;
+    a.t_start = w.rt_start;
+    a.p_start = w.r_start;
+    a.hist = w.sort_hist;
+    a.misc = w.misc;
+    a.rwrap = w.rwrap;
+    a.pflags = w.pflags;
+    a.sorted = w.sorted;
+    a.sort_index = w.sort_index;
+    a.skey = w.skey;
+    a.sflags = w.sflags;
+    for (int p = 0; p < kMaxPasses; ++p) {
+        a.pass = p;
+        a.kin = (p & 1) ? w.keys1 : w.keys0;
+        a.vin = (p & 1) ? w.vals1 : w.vals0;
+        a.kout = (p & 1) ? w.keys0 : w.keys1;
+        a.vout = (p & 1) ? w.vals0 : w.vals1;
+        a.lb = w.sort_lb + (int64_t)p * b.rt_total * kRadix;
+        hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, a);
     }
+    TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
 
