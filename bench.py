@@ -4,8 +4,11 @@
 Workload (one step = one full search): 1M-point 3-D periodic cloud (100^3
 jittered grid in [-1,1]^3, ~50 neighbours, gather, CSR, hashMapLength 2^20),
 inputs resident in HBM; the step runs radiusSearch(..., algorithm='compact',
-format='csr') end to end: structure build, count, scan, one host sync for
-the pair total, output allocation and fill.
+format='csr') end to end: structure build, count, scan and fill (into
+outputs sized from the previous step's total, so the chain runs without a
+host round trip in between), one host sync for the pair total, and the CSR
+tuple.  The roofline figure is the fill pass, timed with HIP events that the
+library records on the search stream during the timed steps.
 
 Prints one JSON line (rank 0).  The CPU baseline is the C/OpenMP oracle
 timed on this host on the same workload.
@@ -13,6 +16,7 @@ timed on this host on the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import statistics
@@ -41,6 +45,31 @@ def fill_bytes(nq: int, dim: int, pairs: int) -> int:
     return nq * (4 * dim + 4) + 8 * (nq + 1) + pairs * 8
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_fill_periodic3d_1M.md")
+
+
+def measured_traffic():
+    """HBM bytes per fill pass from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (line 'traffic_bytes_per_search: N'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_search:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(torch):
+    """A plain device copy of known size (256 MiB read + 256 MiB written)."""
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def cpu_baseline(w, reps: int = 3):
     import oracle
     oracle.build()
@@ -67,6 +96,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=100, help="grid points per dimension (100 -> 1M points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the search does not "
@@ -74,7 +105,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch, synthetic
+    from torchcompactradius_amd import DomainDescription, PointCloud, _lib, radiusSearch, synthetic
 
     w = synthetic.periodic_3d(n=args.n)
     dev = "cuda"
@@ -84,40 +115,42 @@ def main():
                             torch.tensor(w.periodic), 3)
     pc = PointCloud(pos, sup)
 
-    def step(timers=None):
-        from torchcompactradius_amd.search import CloudSpec, run_search
-        if timers is None:
-            return radiusSearch(pc, None, mode=w.mode, domain=dom, hashMapLength=w.hash_map_length,
-                                algorithm="compact", format="csr")
-        spec = CloudSpec(qpos=pos, qsup=sup, rpos=None, rsup=None, domain_min=list(map(float, w.domain_min)),
-                         domain_max=list(map(float, w.domain_max)), periodic=7, mode=w.mode,
-                         hash_map_length=w.hash_map_length)
-        return run_search([spec], want_rows=False, timers=timers)
+    def step():
+        return radiusSearch(pc, None, mode=w.mode, domain=dom, hashMapLength=w.hash_map_length,
+                            algorithm="compact", format="csr")
 
     for _ in range(args.warmup):
         out = step()
     torch.cuda.synchronize()
     P = int(out.indices.numel())
 
+    # the library records HIP events on the search stream around the count
+    # phase and the fill of every search of the timed region
+    L = _lib.load()
+    L.tcr_timing_collect(None, None, 0, None)
+    L.tcr_timing_enable(1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    L.tcr_timing_enable(0)
+    cms = (ctypes.c_float * args.steps)()
+    fms = (ctypes.c_float * args.steps)()
+    got = ctypes.c_int32(0)
+    L.tcr_timing_collect(cms, fms, args.steps, ctypes.byref(got))
     ms = (t1 - t0) * 1e3 / args.steps
     pairs_per_s = P / (ms / 1e3)
-
-    # per-pass HIP-event timing on the search stream (separate loop)
-    timers = {}
-    for _ in range(max(5, args.steps // 2)):
-        step(timers)
-    torch.cuda.synchronize()
-    fill_ms = statistics.median(a.elapsed_time(b) for a, b in timers["fill"])
-    count_ms = statistics.median(a.elapsed_time(b) for a, b in timers["count"])
+    fill = [x for x in fms[:got.value] if x >= 0]
+    count = [x for x in cms[:got.value] if x >= 0]
+    fill_ms = statistics.mean(fill) if fill else float("nan")
+    count_ms = statistics.mean(count) if count else float("nan")
     nq = w.pos.shape[0]
     fb = fill_bytes(nq, 3, P)
     achieved = fb / (fill_ms / 1e3) / 1e9
     e2e_bytes = algorithmic_bytes(nq, 3, P)
+    if args.copy_probe:
+        copy_probe(torch)
 
     line = {
         "metric": METRIC,
@@ -135,8 +168,8 @@ def main():
         "config": {"workload": w.name, "points": nq, "dim": 3, "periodic": "xyz", "mode": w.mode,
                    "format": "csr", "hash_map_length": w.hash_map_length, "pairs": P},
         "roofline": {"bound": "hbm", "kernel": "fill pass", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": fb, "avg_ms": fill_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "algorithmic_bytes": fb, "avg_ms": fill_ms, "launches": len(fill)},
         "end_to_end": {"algorithmic_bytes": e2e_bytes, "GB/s": e2e_bytes / (ms / 1e3) / 1e9,
                        "frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "count_phase_ms": count_ms,
                        "fill_phase_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},

@@ -71,11 +71,12 @@ typedef struct tcr_cloud {
 size_t tcr_search_workspace_size(const tcr_cloud *clouds, int32_t n_clouds);
 
 /* Structure build (periodic wraps, hMax, hCell, cell keys, stable radix sort,
- * cell table, hash table) and the neighbour count pass, then the exclusive
- * scan of the counts into offsets[nq_total + 1] (int64, concatenated over the
- * batch).  Synchronises the stream once at the end and writes the total pair
- * count of the batch to *total_pairs.  qsup/rsup may be NULL when no cloud's
- * mode reads them; for self-search clouds rpos/rsup may be NULL. */
+ * cell table, linear-index lookup table) and the neighbour count pass, then
+ * the exclusive scan of the counts into offsets[nq_total + 1] (int64,
+ * concatenated over the batch).  Synchronises the stream once at the end and
+ * writes the total pair count of the batch to *total_pairs.  qsup/rsup may be
+ * NULL when no cloud's mode reads them; for self-search clouds rpos/rsup may
+ * be NULL. */
 int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds,
                      const float *qpos, const float *qsup,
                      const float *rpos, const float *rsup,
@@ -91,6 +92,27 @@ int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds,
                     const float *rpos, const float *rsup,
                     void *workspace, size_t workspace_bytes,
                     const int64_t *offsets, int64_t *row, int64_t *col, void *stream);
+
+/* The whole search without a host round trip: what tcr_search_count
+ * launches, then the fill of tcr_search_fill into row/col, which hold
+ * `capacity` pairs; asynchronous (no synchronisation).  On the device the fill
+ * is skipped when the total exceeds the capacity: read the total with
+ * tcr_search_finish and, if it is larger, call tcr_search_fill with outputs of
+ * that size.  Replaces the same reference chain as tcr_search_count + _fill
+ * (py/compactHashing/cppSearch.py:179-244), whose host read of the pair total
+ * (cppSearch.py:81-89) sits between the two passes. */
+int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds,
+                      const float *qpos, const float *qsup,
+                      const float *rpos, const float *rsup,
+                      void *workspace, size_t workspace_bytes,
+                      int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
+                      void *stream);
+
+/* Completes tcr_search_launch: synchronises the stream once, reports a
+ * degenerate grid (TCR_ERR_DOMAIN) and writes the total pair count. */
+int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds,
+                      void *workspace, size_t workspace_bytes,
+                      const int64_t *offsets, int64_t *total_pairs, void *stream);
 
 /* Copy one cloud's structure out of the workspace in the reference layout:
  * sort_index int64[nr], sorted_pos fp32[nr, dim], sorted_sup fp32[nr] (NULL to
@@ -134,6 +156,15 @@ int tcr_build_neighbor_list(const int32_t *counts, const int32_t *offsets, int64
                             const float *max_domain, const float *min_domain,
                             const uint8_t *periodicity, int32_t dim, int32_t mode,
                             int64_t *list_i, int64_t *list_j, void *stream);
+
+/* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
+ * (no counterpart in the reference; a benchmark hook).  While enabled, each
+ * launch records HIP events on its stream around the count phase (build,
+ * count, scan) and the fill.  tcr_timing_collect waits for the recorded
+ * events, writes up to max_entries milliseconds per launch (-1 for a phase
+ * that did not run), the number written to *n_entries, and forgets them. */
+int tcr_timing_enable(int32_t on);
+int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int32_t *n_entries);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *tcr_last_error(void);

@@ -1,0 +1,163 @@
+"""GPU parity of the other entry points: batched multi-cloud searches through
+one C-ABI call, radius() with batch ids, and the reference-layout native ops
+(countNeighbors / buildNeighborList / computeHashIndices) -- all against the
+CPU oracle, bit-exact on indices and order."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from torchcompactradius_amd import DomainDescription, PointCloud, native_ops, radius, radiusSearch, synthetic
+from torchcompactradius_amd.search import CloudSpec, run_search
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _cloud(rng, n, dim, per, h):
+    pos = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    sup = np.full(n, h, np.float32) * rng.uniform(0.8, 1.2, n).astype(np.float32)
+    return pos, sup, per
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_batched_clouds_one_launch(seed):
+    """Clouds of uneven size, dimension, mode and periodicity (self and
+    non-self) in one count/fill call; each cloud's rows equal its own
+    search."""
+    rng = np.random.default_rng(300 + seed)
+    modes = ["gather", "scatter", "symmetric", "superSymmetric"]
+    specs, refs = [], []
+    for k in range(9):
+        dim = [2, 3, 1][k % 3]
+        n = int(rng.integers(0 if k == 4 else 50, 3000))
+        per = [bool(rng.integers(0, 2)) for _ in range(dim)]
+        h = {1: 0.02, 2: 0.08, 3: 0.2}[dim]
+        q, hq, _ = _cloud(rng, n, dim, per, h)
+        selfs = k % 2 == 0
+        r, hr = (None, None) if selfs else _cloud(rng, int(rng.integers(1, 2500)), dim, per, h)[:2]
+        mode = modes[k % 4]
+        L = int(rng.choice([1, 97, 4096]))
+        mn, mx = [-1.0] * dim, [1.0] * dim
+        pm = sum(1 << d for d in range(dim) if per[d])
+        specs.append(CloudSpec(qpos=_t(q), qsup=_t(hq), rpos=_t(r), rsup=_t(hr), domain_min=mn, domain_max=mx,
+                               periodic=pm, mode=mode, hash_map_length=L))
+        refs.append(oracle.radius_search(q, hq, r, hr, domain_min=mn, domain_max=mx, periodic=per, mode=mode,
+                                         hash_map_length=L))
+    print(f"case: batched seed={seed}", flush=True)
+    res = run_search(specs, want_rows=True)
+    torch.cuda.synchronize()
+    offs = res.offsets.cpu().numpy()
+    row, col = res.row.cpu().numpy(), res.col.cpu().numpy()
+    q0 = 0
+    for s, o in zip(specs, refs):
+        nq = s.qpos.shape[0]
+        a, b = offs[q0], offs[q0 + nq]
+        np.testing.assert_array_equal(row[a:b], o.row)
+        np.testing.assert_array_equal(col[a:b], o.col)
+        np.testing.assert_array_equal(np.diff(offs[q0:q0 + nq + 1]), o.counts)
+        q0 += nq
+    assert offs[-1] == sum(o.col.shape[0] for o in refs) == res.total
+
+
+def test_radius_with_batch_ids():
+    rng = np.random.default_rng(77)
+    n, m = 2000, 1500
+    x = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    y = rng.uniform(0, 1, (m, 3)).astype(np.float32)
+    bx = np.sort(rng.integers(0, 4, n))
+    by = np.sort(rng.integers(0, 4, m))
+    h = 0.12
+    print("case: radius batch", flush=True)
+    i, j = radius(_t(x), _t(y), h, batch_x=_t(bx), batch_y=_t(by))
+    torch.cuda.synchronize()
+    got = set(zip(i.cpu().tolist(), j.cpu().tolist()))
+    want = set()
+    for b in range(4):
+        qi, ri = np.nonzero(bx == b)[0], np.nonzero(by == b)[0]
+        o = oracle.radius_search(x[qi], np.full(len(qi), h, np.float32), y[ri], np.full(len(ri), h, np.float32),
+                                 mode="gather")
+        want |= set(zip(ri[o.col].tolist(), qi[o.row].tolist()))
+    assert got == want
+
+
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+def test_native_ops_on_exported_structure(mode):
+    """searchNeighbors_cpp's sequence (compactHashing/cppSearch.py:60-98) on
+    the exported structure: countNeighbors -> int32 offsets ->
+    buildNeighborList -> j = sortIndex[j]."""
+    w = synthetic.periodic_3d(n=22)
+    rng = np.random.default_rng(8)
+    sup = (w.sup * rng.uniform(0.7, 1.3, w.sup.shape[0])).astype(np.float32)
+    dom = DomainDescription(torch.tensor(w.domain_min), torch.tensor(w.domain_max), torch.tensor(w.periodic), 3)
+    print(f"case: native ops mode={mode}", flush=True)
+    _, hm = radiusSearch(PointCloud(_t(w.pos), _t(sup)), None, mode=mode, domain=dom, hashMapLength=1009,
+                         algorithm="compact", format="coo", returnStructure=True)
+    q = _t(oracle.wrap(w.pos, w.domain_min, w.domain_max, w.periodic))
+    qs = _t(sup)
+    per = hm.periodicity
+    counts = native_ops.countNeighbors(q, qs, 1, hm.sortedPositions, hm.sortedSupports, hm.hashTable,
+                                       hm.hashMapLength, hm.numCells, hm.sortedCellTable, hm.qMin,
+                                       float(hm.hCell.item()), hm.maxD, hm.minD, per, mode, False)
+    offsets = torch.cat((torch.zeros(1, dtype=torch.int32, device=DEV),
+                         torch.cumsum(counts, 0).to(torch.int32)))[:-1]
+    total = int(offsets[-1].item() + counts[-1].item())
+    i, j = native_ops.buildNeighborList(counts, offsets, total, q, qs, 1, hm.sortedPositions, hm.sortedSupports,
+                                       hm.hashTable, hm.hashMapLength, hm.numCells, hm.sortedCellTable, hm.qMin,
+                                       float(hm.hCell.item()), hm.maxD, hm.minD, per, mode, False)
+    j = hm.sortIndex[j]
+    torch.cuda.synchronize()
+    o = oracle.radius_search(w.pos, sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode=mode, hash_map_length=1009)
+    np.testing.assert_array_equal(counts.cpu().numpy(), o.counts)
+    np.testing.assert_array_equal(i.cpu().numpy(), o.row)
+    np.testing.assert_array_equal(j.cpu().numpy(), o.col)
+
+
+def test_native_ops_errors():
+    x = torch.zeros((4, 3), device=DEV)
+    with pytest.raises(RuntimeError):
+        native_ops.countNeighbors(x, None, 1, x, None, torch.zeros((8, 2), dtype=torch.int32, device=DEV), 8,
+                                  torch.ones(3, dtype=torch.int32, device=DEV),
+                                  torch.zeros((1, 3), dtype=torch.int32, device=DEV), None, 0.1,
+                                  torch.ones(3, device=DEV), torch.zeros(3, device=DEV),
+                                  torch.zeros(3, dtype=torch.bool, device=DEV), "gather", False)
+    with pytest.raises(RuntimeError):
+        native_ops.computeHashIndices(torch.zeros((3, 4), dtype=torch.int32, device=DEV), 16)
+
+
+def test_compute_hash_indices():
+    rng = np.random.default_rng(21)
+    for dim in (1, 2, 3):
+        cells = rng.integers(-2**31, 2**31 - 1, (500, dim), dtype=np.int64).astype(np.int32)
+        cells[:100] = rng.integers(-5, 50, (100, dim)).astype(np.int32)
+        for L in (1, 7, 4096, 2**20):
+            print(f"case: hash dim={dim} L={L}", flush=True)
+            got = native_ops.computeHashIndices(_t(cells), L).cpu().numpy()
+            want = np.array([oracle.hash_index(c, L) for c in cells], dtype=np.int64)
+            np.testing.assert_array_equal(got.astype(np.int64), want)
+
+
+def test_speculative_capacity_paths():
+    """Repeated searches of one shape launch count and fill back to back into
+    outputs sized from the previous total; a denser cloud of the same shape
+    overflows that guess and takes the separate fill, a sparser one fits."""
+    rng = np.random.default_rng(31)
+    q = rng.uniform(-1, 1, (3000, 3)).astype(np.float32)
+    mn, mx = [-1.0] * 3, [1.0] * 3
+    for h in (0.15, 0.15, 0.25, 0.1, 0.1):
+        sup = np.full(3000, h, np.float32)
+        print(f"case: speculative h={h}", flush=True)
+        dom = DomainDescription(torch.tensor(mn), torch.tensor(mx), torch.tensor([True, False, True]), 3)
+        g = radiusSearch(PointCloud(_t(q), _t(sup)), None, mode="gather", domain=dom, hashMapLength=4096,
+                         algorithm="compact", format="coo")
+        torch.cuda.synchronize()
+        o = oracle.radius_search(q, sup, domain_min=mn, domain_max=mx, periodic=[True, False, True], mode="gather",
+                                 hash_map_length=4096)
+        np.testing.assert_array_equal(g.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(g.col.cpu().numpy(), o.col)

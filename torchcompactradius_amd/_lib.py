@@ -46,7 +46,8 @@ _lib: Optional[ctypes.CDLL] = None
 
 # Every symbol include/tcr.h declares (checked by the ABI tests).
 EXPORTED = (
-    "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_export",
+    "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_launch",
+    "tcr_search_finish", "tcr_search_export", "tcr_timing_enable", "tcr_timing_collect",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_last_error", "tcr_version",
 )
@@ -75,6 +76,14 @@ def load() -> ctypes.CDLL:
                                    ctypes.POINTER(ctypes.c_int64), vp]
     L.tcr_search_fill.restype = ctypes.c_int
     L.tcr_search_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.tcr_search_launch.restype = ctypes.c_int
+    L.tcr_search_launch.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i64, vp]
+    L.tcr_timing_enable.restype = ctypes.c_int
+    L.tcr_timing_enable.argtypes = [i32]
+    L.tcr_timing_collect.restype = ctypes.c_int
+    L.tcr_timing_collect.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int32)]
+    L.tcr_search_finish.restype = ctypes.c_int
+    L.tcr_search_finish.argtypes = [cl, i32, vp, ctypes.c_size_t, vp, ctypes.POINTER(ctypes.c_int64), vp]
     L.tcr_search_export.restype = ctypes.c_int
     L.tcr_search_export.argtypes = [cl, i32, i32, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tcr_compute_hash_indices.restype = ctypes.c_int

@@ -1,4 +1,5 @@
 // C ABI of libtcr_hip.so (declared in include/tcr.h).
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -9,9 +10,6 @@ namespace tcr {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
-
-int query_count(const Batch &b, Work &w, hipStream_t s);
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, hipStream_t s);
 
 static int64_t tiles_of(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
@@ -67,16 +65,43 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) 
     return true;
 }
 
-// Bytes of the header block: the descriptors followed by the five prefix
-// arrays (q_start, r_start, rt_start, hb_start, lt_start), contiguous so that
-// one host-to-device copy uploads them.
-static size_t header_bytes(int64_t B) { return sizeof(CloudDev) * B + 5 * sizeof(int64_t) * (B + 1); }
+// Bytes of the header block: the descriptors, the five prefix arrays
+// (q_start, r_start, rt_start, hb_start, lt_start) and the zeroed device
+// counters, contiguous so that one upload writes them all.
+static size_t header_bytes(int64_t B) {
+    return sizeof(CloudDev) * B + 5 * sizeof(int64_t) * (B + 1) + sizeof(uint32_t) * kMiscWords;
+}
 
 static void header_views(char *base, int64_t B, Work &w) {
     w.desc = (CloudDev *)base;
     int64_t *pre = base ? (int64_t *)(base + sizeof(CloudDev) * B) : nullptr;
     int64_t **dst[5] = {&w.q_start, &w.r_start, &w.rt_start, &w.hb_start, &w.lt_start};
     for (int k = 0; k < 5; ++k) *dst[k] = pre ? pre + k * (B + 1) : nullptr;
+    w.misc = pre ? (uint32_t *)(pre + 5 * (B + 1)) : nullptr;
+}
+
+// Upload of the header through kernel arguments: no host staging buffer to
+// keep alive, no pageable-copy synchronisation, and the counters are zeroed
+// in the same launch.  Chunks of kUploadBytes per launch.
+constexpr int kUploadBytes = 2048;
+struct UploadChunk {
+    uint32_t data[kUploadBytes / 4];
+};
+__global__ void __launch_bounds__(256) k_upload(UploadChunk c, uint32_t *__restrict__ dst, uint32_t nwords) {
+    for (uint32_t i = threadIdx.x; i < nwords; i += 256) dst[i] = c.data[i];
+}
+
+static int upload_header(const std::vector<char> &img, void *dst, hipStream_t s) {
+    const size_t n = img.size();  // a multiple of 4 bytes
+    for (size_t off = 0; off < n; off += kUploadBytes) {
+        UploadChunk c;
+        const size_t len = std::min<size_t>(kUploadBytes, n - off);
+        std::memcpy(c.data, img.data() + off, len);
+        hipLaunchKernelGGL(k_upload, dim3(1), dim3(256), 0, s, c, (uint32_t *)((char *)dst + off),
+                           (uint32_t)(len / 4));
+    }
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
 }
 
 size_t carve_work(const Batch &b, void *ws, Work &w) {
@@ -84,7 +109,6 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
     Carve cv(ws);
     const int64_t B = b.B, nr = b.nr_total, nq = b.nq_total, hb = b.hb_total;
     header_views(cv.take<char>(header_bytes(B)), B, w);
-    w.misc = cv.take<uint32_t>(kMiscWords);
     w.rwrap = cv.take<float4>(nr);
     w.keys0 = cv.take<uint32_t>(nr);
     w.keys1 = cv.take<uint32_t>(nr);
@@ -171,9 +195,96 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
     }
     if (upload) {
         std::memcpy(hv.desc, b.dev.data(), sizeof(CloudDev) * b.B);
-        // the host image outlives the stream synchronisation of tcr_search_count
-        TCR_HIP(hipMemcpyAsync(p.w.desc, p.header.data(), p.header.size(), hipMemcpyHostToDevice, s));
+        return upload_header(p.header, p.w.desc, s);
     }
+    return TCR_OK;
+}
+
+// Per-thread pinned readback slots (total pair count, error word).
+struct Readback {
+    int64_t total;
+    uint32_t err;
+};
+static Readback *readback_slot() {
+    static thread_local Readback *slot = nullptr;
+    if (!slot && hipHostMalloc((void **)&slot, sizeof(Readback), hipHostMallocDefault) != hipSuccess) slot = nullptr;
+    return slot;
+}
+
+// Optional per-thread phase timing: HIP events recorded on the search
+// stream around the count phase (build + count + scan) and the fill phase.
+struct PhaseEvents {
+    hipEvent_t e[4];
+    bool has_fill;
+};
+struct Timing {
+    bool on = false;
+    std::vector<PhaseEvents> pending;
+};
+static thread_local Timing g_timing;
+
+static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
+    if (hipEventCreate(&pe.e[k]) != hipSuccess) return nullptr;
+    hipEventRecord(pe.e[k], s);
+    return pe.e[k];
+}
+
+// Everything of a search up to the scan, and the fill when capacity >= 0
+// (skipped on the device if the total exceeds it).  Asynchronous.
+static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup,
+                         const float *rpos, const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets,
+                         int64_t *row, int64_t *col, int64_t capacity, hipStream_t s, Prepared &p) {
+    int rc = prepare(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, s, p, true);
+    if (rc) return rc;
+    if (!offsets) {
+        set_error("offsets missing");
+        return TCR_ERR_ARG;
+    }
+    PhaseEvents pe{};
+    const bool timed = g_timing.on;
+    if (timed) timing_event(pe, 0, s);
+    rc = build_structure(p.b, p.w, s);
+    if (rc) return rc;
+    rc = query_count(p.b, p.w, s);
+    if (rc) return rc;
+    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
+    if (rc) return rc;
+    if (timed) timing_event(pe, 1, s);
+    if (capacity >= 0) {
+        if (!col) {
+            set_error("col missing");
+            return TCR_ERR_ARG;
+        }
+        if (timed) timing_event(pe, 2, s);
+        rc = query_fill(p.b, p.w, offsets, row, col, capacity, s);
+        if (timed) timing_event(pe, 3, s);
+        pe.has_fill = true;
+    }
+    if (timed) g_timing.pending.push_back(pe);
+    return rc;
+}
+
+// One synchronisation: total pair count and the degenerate-grid flag.
+static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *total_pairs, hipStream_t s) {
+    Readback *rb = readback_slot();
+    if (!rb) {
+        set_error("hipHostMalloc of the readback slot failed");
+        return TCR_ERR_HIP;
+    }
+    TCR_HIP(hipMemcpyAsync(&rb->total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(&rb->err, p.w.misc + kMiscErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    if (rb->err) {
+        std::vector<CloudDev> dv(p.b.B);
+        TCR_HIP(hipMemcpy(dv.data(), p.w.desc, sizeof(CloudDev) * p.b.B, hipMemcpyDeviceToHost));
+        for (int32_t c = 0; c < p.b.B; ++c)
+            if (dv[c].err) {
+                set_error("cloud " + std::to_string(c) +
+                          ": a periodic dimension has no cells (support exceeds the domain)");
+                return TCR_ERR_DOMAIN;
+            }
+    }
+    if (total_pairs) *total_pairs = rb->total;
     return TCR_OK;
 }
 
@@ -202,27 +313,57 @@ int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds, const float *qpo
                      int64_t *offsets, int64_t *total_pairs, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     Prepared p;
-    int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, true);
+    int rc = launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, nullptr,
+                           nullptr, -1, s, p);
     if (rc) return rc;
-    rc = build_structure(p.b, p.w, s);
-    if (rc) return rc;
-    rc = query_count(p.b, p.w, s);
-    if (rc) return rc;
-    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
-    if (rc) return rc;
-    // one synchronisation: total pair count plus the per-cloud error flags
-    std::vector<CloudDev> dv(p.b.B);
-    int64_t total = 0;
-    TCR_HIP(hipMemcpyAsync(&total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    TCR_HIP(hipMemcpyAsync(dv.data(), p.w.desc, sizeof(CloudDev) * p.b.B, hipMemcpyDeviceToHost, s));
-    TCR_HIP(hipStreamSynchronize(s));
-    for (int32_t c = 0; c < p.b.B; ++c)
-        if (dv[c].err) {
-            set_error("cloud " + std::to_string(c) + ": a periodic dimension has no cells (support exceeds the domain)");
-            return TCR_ERR_DOMAIN;
-        }
-    if (total_pairs) *total_pairs = total;
+    return finish_search(p, offsets, total_pairs, s);
+}
+
+int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
+                      const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
+                      int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity, void *stream) {
+    Prepared p;
+    if (capacity < 0) {
+        set_error("capacity must be >= 0");
+        return TCR_ERR_ARG;
+    }
+    return launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, row, col,
+                         capacity, (hipStream_t)stream, p);
+}
+
+int tcr_timing_enable(int32_t on) {
+    g_timing.on = on != 0;
     return TCR_OK;
+}
+
+int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int32_t *n_entries) {
+    int32_t k = 0;
+    for (PhaseEvents &pe : g_timing.pending) {
+        if (k < max_entries) {
+            float c = -1.0f, f = -1.0f;
+            if (pe.e[1] && hipEventSynchronize(pe.e[1]) == hipSuccess) hipEventElapsedTime(&c, pe.e[0], pe.e[1]);
+            if (pe.has_fill && pe.e[3] && hipEventSynchronize(pe.e[3]) == hipSuccess)
+                hipEventElapsedTime(&f, pe.e[2], pe.e[3]);
+            if (count_ms) count_ms[k] = c;
+            if (fill_ms) fill_ms[k] = f;
+            ++k;
+        }
+        for (hipEvent_t e : pe.e)
+            if (e) hipEventDestroy(e);
+    }
+    g_timing.pending.clear();
+    if (n_entries) *n_entries = k;
+    return TCR_OK;
+}
+
+int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds, void *workspace, size_t workspace_bytes,
+                      const int64_t *offsets, int64_t *total_pairs, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    Prepared p;
+    int rc = prepare(clouds, n_clouds, nullptr, nullptr, nullptr, nullptr, workspace, workspace_bytes, s, p, false,
+                     false);
+    if (rc) return rc;
+    return finish_search(p, offsets, total_pairs, s);
 }
 
 int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
@@ -232,7 +373,7 @@ int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos
     Prepared p;
     int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false);
     if (rc) return rc;
-    return query_fill(p.b, p.w, offsets, row, col, s);
+    return query_fill(p.b, p.w, offsets, row, col, INT64_MAX, s);
 }
 
 __global__ void k_export_sort_index(const int32_t *__restrict__ si, int64_t n, int64_t *__restrict__ out) {

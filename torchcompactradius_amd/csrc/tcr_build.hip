@@ -144,14 +144,21 @@ __global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, in
             }
         }
     }
-    // last block: grids of every cloud
-    __threadfence();
-    if (threadIdx.x == 0) s_last = atomicAdd(&misc[kMiscPrepTicket], 1u) == gridDim.x - 1;
+    // last block: grids of every cloud.  One release fence per block after
+    // the barrier orders the block's atomicMax updates before its ticket
+    // (a fence per thread would write back the L2 once per wavefront).
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&misc[kMiscPrepTicket], 1u) == gridDim.x - 1;
+        if (s_last) __threadfence();
+    }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
-    for (int c = threadIdx.x; c < B; c += kBlock)
+    for (int c = threadIdx.x; c < B; c += kBlock) {
         grid_of(desc[c], __hip_atomic_load(&desc[c].hmax_ord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (desc[c].err) atomicOr(&misc[kMiscErr], 1u);
+    }
 }
 
 // Reference set: second wrap (cppSearch.py:221) on top of radiusSearch's
@@ -363,8 +370,7 @@ static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)
 
 int build_structure(const Batch &b, Work &w, hipStream_t s) {
     const int32_t B = b.B;
-    int rc = fill_u32(w.misc, 0u, kMiscWords, s);
-    if (rc) return rc;
+    int rc = TCR_OK;
     // hMax + grid: one launch
     bool ref_side = false;
     for (const CloudDev &d : b.dev)

@@ -130,6 +130,8 @@ struct GroupArgs {
     uint64_t *masks;             // [nr * kSmallRounds]
     int32_t *counts;             // [nq] per global query index
     const int64_t *offsets;      // [nq+1] (fill)
+    const int64_t *total;        // offsets + nq: the fill runs only if *total <= capacity
+    int64_t capacity;
     int64_t *soff;               // [nr] row offset of each sorted slot's query (fill)
     int64_t *row, *col;          // fill outputs (row may be null)
     int32_t *cursor;             // [nq] per-query write cursor across chunks (fill)
@@ -683,6 +685,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
 __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
+    if (*a.total > a.capacity) return;
     const CellRange cr = my_cells(a.misc[kMiscCells]);
     if (cr.first >= cr.end) return;
     CellLoad cur = load_cell(a, cr.first);
@@ -704,6 +707,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
 
 template <int DIM, int MODE, bool FILL>
 __global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a) {
+    if (FILL && *a.total > a.capacity) return;
     const uint32_t n = *a.big_count;
     const uint32_t wpb = kGroupThreads / kWave;
     for (uint32_t i = blockIdx.x * wpb + threadIdx.x / kWave; i < n; i += gridDim.x * wpb) {
@@ -762,10 +766,11 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
     return TCR_OK;
 }
 
-int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
-                   bool list, hipStream_t s);
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
+                   bool fill, bool list, hipStream_t s);
 
-static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64_t *col) {
+static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
+                            int64_t capacity) {
     GroupArgs ga;
     ga.desc = w.desc;
     ga.r_start = w.r_start;
@@ -783,6 +788,8 @@ static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64
     ga.masks = w.masks;
     ga.counts = w.counts;
     ga.offsets = offsets;
+    ga.total = offsets ? offsets + b.nq_total : nullptr;
+    ga.capacity = capacity;
     ga.soff = w.soff;
     ga.row = row;
     ga.col = col;
@@ -797,8 +804,8 @@ static GroupArgs group_args(Work &w, const int64_t *offsets, int64_t *row, int64
 // Count or fill pass over the batch: wavefront-per-cell kernels for the
 // self-search clouds (their strays collected by the count pass), the
 // per-query kernel for every other cloud and for the strays.
-static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
-                      hipStream_t s) {
+static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
+                      int64_t capacity, bool fill, hipStream_t s) {
     if (b.nq_total == 0) return TCR_OK;
     bool any_group = false, any_other = false;
     for (const CloudDev &d : b.dev) {
@@ -807,10 +814,10 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
         else any_other = true;
     }
     if (any_group) {
-        const GroupArgs ga = group_args(w, offsets, row, col);
+        const GroupArgs ga = group_args(b, w, offsets, row, col, capacity);
         if (!fill) {
             // misc[kMiscStrays] and misc[kMiscBig] start at zero: the
-            // build clears all of misc
+            // header upload clears all of misc
             hipLaunchKernelGGL(k_stencil, dim3(2048), dim3(256), 0, s, ga);
         } else {
             hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((b.nr_total + 255) / 256)), dim3(256), 0, s,
@@ -820,16 +827,19 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
         }
         int rc = fill ? launch_modes<true>(b, ga, s) : launch_modes<false>(b, ga, s);
         if (rc) return rc;
-        rc = launch_generic(b, w, offsets, row, col, fill, true, s);
+        rc = launch_generic(b, w, offsets, row, col, capacity, fill, true, s);
         if (rc) return rc;
     }
-    if (any_other) return launch_generic(b, w, offsets, row, col, fill, false, s);
+    if (any_other) return launch_generic(b, w, offsets, row, col, capacity, fill, false, s);
     return TCR_OK;
 }
 
-int query_count(const Batch &b, Work &w, hipStream_t s) { return query_pass(b, w, nullptr, nullptr, nullptr, false, s); }
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, hipStream_t s) {
-    return query_pass(b, w, offsets, row, col, true, s);
+int query_count(const Batch &b, Work &w, hipStream_t s) {
+    return query_pass(b, w, nullptr, nullptr, nullptr, -1, false, s);
+}
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
+               hipStream_t s) {
+    return query_pass(b, w, offsets, row, col, capacity, true, s);
 }
 
 }  // namespace tcr

@@ -29,6 +29,7 @@ enum : int {
     kMiscPasses = 3,     // radix passes of this search
     kMiscPrepTicket = 4, // last-block ticket of k_prep
     kMiscCellTicket = 5, // tile ticket of k_cells
+    kMiscErr = 6,        // nonzero: some cloud has a degenerate grid
     kMiscSortTicket = 8, // tile tickets of the radix passes (kMaxPasses)
     kMiscWords = 16,
 };
@@ -160,8 +161,14 @@ int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64
                    hipStream_t s);
 size_t scan_partials_needed(int64_t n);
 
-// Structure build (hMax, grid, keys, sort, cells, lin table).
+// Structure build (hMax, grid, keys, sort, cells, lin table).  Expects the
+// header block (descriptors, prefixes, zeroed misc) uploaded.
 int build_structure(const Batch &b, Work &w, hipStream_t s);
+// Count pass (counts[] and the fill's masks); fill pass into offsets,
+// skipped on the device when offsets[nq_total] > capacity.
+int query_count(const Batch &b, Work &w, hipStream_t s);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
+               hipStream_t s);
 // Reference-layout bucket tables of every cloud (for export).
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted

@@ -154,6 +154,8 @@ struct GenericArgs {
     const int32_t *sort_index;
     int32_t *counts;
     const int64_t *offsets;
+    const int64_t *total;       // offsets + nq: the fill runs only if *total <= capacity
+    int64_t capacity;
     int64_t *row, *col;
     const int32_t *list;        // stray list (list mode)
     const uint32_t *list_count;
@@ -201,6 +203,7 @@ __device__ __forceinline__ bool owned_by_cells(const CloudDev &cd) {
 // Every query of the clouds the wavefront-per-cell kernels do not own.
 template <bool FILL>
 __global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
+    if (FILL && *a.total > a.capacity) return;
     const int64_t g = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
     if (g >= a.nq) return;
     const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
@@ -211,13 +214,14 @@ __global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
 // The stray list left by the wavefront-per-cell count pass.
 template <bool FILL>
 __global__ void __launch_bounds__(kQBlock) k_query_list(GenericArgs a) {
+    if (FILL && *a.total > a.capacity) return;
     const int64_t n = *a.list_count;
     for (int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kQBlock)
         generic_query<FILL>(a, a.list[i]);
 }
 
-int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, bool fill,
-                   bool list, hipStream_t s) {
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
+                   bool fill, bool list, hipStream_t s) {
     GenericArgs a;
     a.desc = w.desc;
     a.B = b.B;
@@ -231,6 +235,8 @@ int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row
     a.sort_index = w.sort_index;
     a.counts = w.counts;
     a.offsets = offsets;
+    a.total = offsets ? offsets + b.nq_total : nullptr;
+    a.capacity = capacity;
     a.row = row;
     a.col = col;
     a.list = w.stray_list;
