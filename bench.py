@@ -89,6 +89,70 @@ def cpu_baseline(w, reps: int = 3):
                       f"{t * 1e3:.0f} ms each"}
 
 
+def bench_batched(torch, reps: int = 5, scale: float = 1.0):
+    """Batched workload: 256 independent clouds (log-uniform 1e3..2e5 points,
+    half 2-D half 3-D, per-cloud support and periodicity).  Compares one
+    radiusSearch per cloud (plain per-request launches) with the whole batch
+    in one launch chain, autotuned and replayed from a captured HIP graph;
+    checks that every cloud's CSR output is identical."""
+    from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch, synthetic
+    from torchcompactradius_amd.autotune import autotune
+    from torchcompactradius_amd.search import CloudSpec, SearchPlan, periodic_mask
+
+    ws = synthetic.batched_clouds(scale=scale)
+    dev = "cuda"
+    data = [(torch.from_numpy(w.pos).to(dev), torch.from_numpy(w.sup).to(dev)) for w in ws]
+    doms = [DomainDescription(torch.tensor(w.domain_min), torch.tensor(w.domain_max), torch.tensor(w.periodic),
+                              w.pos.shape[1]) for w in ws]
+
+    def plain():
+        return [radiusSearch(PointCloud(p, h), None, mode=w.mode, domain=d, hashMapLength=w.hash_map_length,
+                             algorithm="compact", format="csr") for (p, h), w, d in zip(data, ws, doms)]
+
+    outs = plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs = plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    P = sum(int(o.indices.numel()) for o in outs)
+    N = sum(w.pos.shape[0] for w in ws)
+
+    specs = [CloudSpec(qpos=p, qsup=h, rpos=None, rsup=None, domain_min=[float(x) for x in w.domain_min],
+                       domain_max=[float(x) for x in w.domain_max], periodic=periodic_mask(w.periodic, len(w.periodic)),
+                       mode=w.mode, hash_map_length=w.hash_map_length) for (p, h), w in zip(data, ws)]
+    res = {}
+    for name, dt in (("int64", torch.int64), ("int32", torch.int32)):
+        plan = SearchPlan(specs, want_rows=False, index_dtype=dt)
+        tuning = autotune(plan)
+        r = plan.run()
+        plan.capture(r.total + r.total // 32 + 256)
+        r = plan.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = plan.replay()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        offs = r.offsets.cpu()
+        exact = r.total == P
+        q0 = 0
+        for o, w in zip(outs, ws):
+            n = w.pos.shape[0]
+            a, b = int(offs[q0]), int(offs[q0 + n])
+            exact = exact and torch.equal(r.col[a:b].to(torch.int64), o.indices) and \
+                torch.equal((offs[q0:q0 + n + 1] - a).to(torch.int32), o.indptr.cpu())
+            q0 += n
+        res[name] = {"ms": t * 1e3, "pairs_per_s": P / t, "bit_exact_vs_plain": bool(exact), "tuning": list(tuning)}
+        plan.release_graph()
+        del plan, r
+    return {"workload": f"batched-{len(ws)}-clouds", "clouds": len(ws), "points": N, "pairs": P,
+            "plain_ms": t_plain * 1e3, "plain_pairs_per_s": P / t_plain,
+            "graph_int64": res["int64"], "graph_int32": res["int32"],
+            "speedup_vs_plain": t_plain / (res["int64"]["ms"] / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +160,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=100, help="grid points per dimension (100 -> 1M points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (256-cloud) comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -174,6 +239,8 @@ def main():
                        "frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "count_phase_ms": count_ms,
                        "fill_phase_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},
     }
+    if not args.no_batched:
+        line["batched"] = bench_batched(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

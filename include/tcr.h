@@ -86,12 +86,14 @@ int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds,
 /* Pair fill after tcr_search_count with the same arguments: row (may be NULL)
  * gets the query index, col the original reference index, both local to
  * their cloud, at offsets[q] .. offsets[q+1] for query q, in the reference's
- * order (cell stencil order, then sorted reference order). */
+ * order (cell stencil order, then sorted reference order).  index_bytes is
+ * 8 (int64 entries, the reference's dtype) or 4 (int32 entries). */
 int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds,
                     const float *qpos, const float *qsup,
                     const float *rpos, const float *rsup,
                     void *workspace, size_t workspace_bytes,
-                    const int64_t *offsets, int64_t *row, int64_t *col, void *stream);
+                    const int64_t *offsets, void *row, void *col, int32_t index_bytes,
+                    void *stream);
 
 /* The whole search without a host round trip: what tcr_search_count
  * launches, then the fill of tcr_search_fill into row/col, which hold
@@ -105,8 +107,34 @@ int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds,
                       const float *qpos, const float *qsup,
                       const float *rpos, const float *rsup,
                       void *workspace, size_t workspace_bytes,
-                      int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
-                      void *stream);
+                      int64_t *offsets, void *row, void *col, int32_t index_bytes,
+                      int64_t capacity, void *stream);
+
+/* The launches of tcr_search_launch captured once into a HIP graph (on a
+ * created stream, not the null stream) and replayed by tcr_graph_launch: the
+ * replay reads the same input buffers, workspace and outputs, so a caller
+ * that rewrites positions/supports in place between replays searches the new
+ * values with one launch.  Complete each replay with tcr_search_finish.
+ * *graph receives an opaque handle for tcr_graph_launch / _destroy.  No
+ * counterpart in the reference (its chain is host-driven torch ops). */
+int tcr_graph_capture(const tcr_cloud *clouds, int32_t n_clouds,
+                      const float *qpos, const float *qsup,
+                      const float *rpos, const float *rsup,
+                      void *workspace, size_t workspace_bytes,
+                      int64_t *offsets, void *row, void *col, int32_t index_bytes,
+                      int64_t capacity, void **graph, void *stream);
+int tcr_graph_launch(void *graph, void *stream);
+int tcr_graph_destroy(void *graph);
+
+/* Launch shape of the wavefront-per-cell kernels, process-wide (set it
+ * between searches, not while another thread launches one); 0 = automatic.
+ * Picked per batch shape by the package's autotuner. */
+typedef struct tcr_tuning {
+    int32_t cell_workgroups;     /* workgroups of the count/fill cell kernels (rounded up to a multiple of 8) */
+    int32_t stencil_workgroups;  /* workgroups of the stencil resolution kernel */
+} tcr_tuning;
+int tcr_set_tuning(const tcr_tuning *tuning);  /* NULL restores the defaults */
+int tcr_get_tuning(tcr_tuning *tuning);
 
 /* Completes tcr_search_launch: synchronises the stream once, reports a
  * degenerate grid (TCR_ERR_DOMAIN) and writes the total pair count. */

@@ -161,3 +161,62 @@ def test_speculative_capacity_paths():
                                  hash_map_length=4096)
         np.testing.assert_array_equal(g.row.cpu().numpy(), o.row)
         np.testing.assert_array_equal(g.col.cpu().numpy(), o.col)
+
+
+def _specs_and_refs(rng, clouds):
+    specs, refs = [], []
+    for q, h, per, mode in clouds:
+        dim = q.shape[1]
+        mn, mx = [-1.0] * dim, [1.0] * dim
+        pm = sum(1 << d for d in range(dim) if per[d])
+        specs.append(CloudSpec(qpos=_t(q), qsup=_t(h), rpos=None, rsup=None, domain_min=mn, domain_max=mx,
+                               periodic=pm, mode=mode, hash_map_length=4096))
+        refs.append(oracle.radius_search(q, h, domain_min=mn, domain_max=mx, periodic=per, mode=mode,
+                                         hash_map_length=4096, want_rows=False))
+    return specs, refs
+
+
+def _check_batch(res, specs, refs):
+    offs = res.offsets.cpu().numpy()
+    col = res.col.cpu().numpy().astype(np.int64)
+    q0 = 0
+    for s, o in zip(specs, refs):
+        nq = s.qpos.shape[0]
+        np.testing.assert_array_equal(col[offs[q0]:offs[q0 + nq]], o.col)
+        np.testing.assert_array_equal(np.diff(offs[q0:q0 + nq + 1]), o.counts)
+        q0 += nq
+
+
+def test_graph_replay_int32_and_tuning():
+    """A captured graph replays the whole chain; positions rewritten in place
+    between replays are searched anew; int32 outputs and non-default launch
+    shapes give the same pairs."""
+    from torchcompactradius_amd.search import SearchPlan
+    rng = np.random.default_rng(55)
+    base = []
+    for k in range(5):
+        dim = 2 + k % 2
+        n = int(rng.integers(500, 4000))
+        q = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+        h = np.full(n, 0.12 if dim == 2 else 0.25, np.float32)
+        base.append([q, h, [bool(rng.integers(0, 2)) for _ in range(dim)], ["gather", "symmetric"][k % 2]])
+    specs, refs = _specs_and_refs(rng, base)
+    for tuning in (None, (8, 64), (1024, 4096)):
+        for dt in (torch.int64, torch.int32):
+            print(f"case: graph tuning={tuning} dtype={dt}", flush=True)
+            plan = SearchPlan(specs, want_rows=False, index_dtype=dt)
+            plan.tuning = tuning
+            r = plan.run()
+            assert r.col.dtype == dt
+            _check_batch(r, specs, refs)
+            plan.capture(r.total + 64)
+            _check_batch(plan.replay(), specs, refs)
+            # move every point a little, rewrite the plan's buffer in place, replay
+            moved = []
+            for q, h, per, mode in base:
+                q2 = (q + rng.uniform(-0.01, 0.01, q.shape)).astype(np.float32)
+                moved.append([q2, h, per, mode])
+            specs2, refs2 = _specs_and_refs(rng, moved)
+            plan.qpos.copy_(torch.cat([s.qpos.reshape(-1) for s in specs2]))
+            _check_batch(plan.replay(), specs2, refs2)
+            plan.release_graph()

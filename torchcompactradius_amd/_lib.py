@@ -36,6 +36,11 @@ class TcrCloud(ctypes.Structure):
     ]
 
 
+class TcrTuning(ctypes.Structure):
+    """Mirror of ``tcr_tuning`` in include/tcr.h."""
+    _fields_ = [("cell_workgroups", ctypes.c_int32), ("stencil_workgroups", ctypes.c_int32)]
+
+
 class TcrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libtcr_hip error {code}: {msg}")
@@ -48,6 +53,7 @@ _lib: Optional[ctypes.CDLL] = None
 EXPORTED = (
     "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_launch",
     "tcr_search_finish", "tcr_search_export", "tcr_timing_enable", "tcr_timing_collect",
+    "tcr_graph_capture", "tcr_graph_launch", "tcr_graph_destroy", "tcr_set_tuning", "tcr_get_tuning",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_last_error", "tcr_version",
 )
@@ -75,9 +81,20 @@ def load() -> ctypes.CDLL:
     L.tcr_search_count.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp,
                                    ctypes.POINTER(ctypes.c_int64), vp]
     L.tcr_search_fill.restype = ctypes.c_int
-    L.tcr_search_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.tcr_search_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp]
     L.tcr_search_launch.restype = ctypes.c_int
-    L.tcr_search_launch.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i64, vp]
+    L.tcr_search_launch.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, i64, vp]
+    L.tcr_graph_capture.restype = ctypes.c_int
+    L.tcr_graph_capture.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, i64,
+                                    ctypes.POINTER(ctypes.c_void_p), vp]
+    L.tcr_graph_launch.restype = ctypes.c_int
+    L.tcr_graph_launch.argtypes = [vp, vp]
+    L.tcr_graph_destroy.restype = ctypes.c_int
+    L.tcr_graph_destroy.argtypes = [vp]
+    L.tcr_set_tuning.restype = ctypes.c_int
+    L.tcr_set_tuning.argtypes = [ctypes.POINTER(TcrTuning)]
+    L.tcr_get_tuning.restype = ctypes.c_int
+    L.tcr_get_tuning.argtypes = [ctypes.POINTER(TcrTuning)]
     L.tcr_timing_enable.restype = ctypes.c_int
     L.tcr_timing_enable.argtypes = [i32]
     L.tcr_timing_collect.restype = ctypes.c_int

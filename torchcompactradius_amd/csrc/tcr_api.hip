@@ -233,7 +233,7 @@ static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
 // (skipped on the device if the total exceeds it).  Asynchronous.
 static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup,
                          const float *rpos, const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets,
-                         int64_t *row, int64_t *col, int64_t capacity, hipStream_t s, Prepared &p) {
+                         PairOut out, int64_t capacity, hipStream_t s, Prepared &p, bool allow_timing = true) {
     int rc = prepare(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, s, p, true);
     if (rc) return rc;
     if (!offsets) {
@@ -241,7 +241,7 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
         return TCR_ERR_ARG;
     }
     PhaseEvents pe{};
-    const bool timed = g_timing.on;
+    const bool timed = g_timing.on && allow_timing;
     if (timed) timing_event(pe, 0, s);
     rc = build_structure(p.b, p.w, s);
     if (rc) return rc;
@@ -251,12 +251,12 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     if (rc) return rc;
     if (timed) timing_event(pe, 1, s);
     if (capacity >= 0) {
-        if (!col) {
+        if (!out.col) {
             set_error("col missing");
             return TCR_ERR_ARG;
         }
         if (timed) timing_event(pe, 2, s);
-        rc = query_fill(p.b, p.w, offsets, row, col, capacity, s);
+        rc = query_fill(p.b, p.w, offsets, out, capacity, s);
         if (timed) timing_event(pe, 3, s);
         pe.has_fill = true;
     }
@@ -288,6 +288,18 @@ static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *tot
     return TCR_OK;
 }
 
+static bool pair_out(void *row, void *col, int32_t index_bytes, PairOut &out) {
+    if (index_bytes != 4 && index_bytes != 8) {
+        set_error("index_bytes must be 4 or 8");
+        return false;
+    }
+    out = PairOut{row, col, index_bytes == 4 ? 1 : 0};
+    return true;
+}
+
+static Tuning g_tuning;
+Tuning &tuning() { return g_tuning; }
+
 }  // namespace tcr
 
 using namespace tcr;
@@ -313,22 +325,118 @@ int tcr_search_count(const tcr_cloud *clouds, int32_t n_clouds, const float *qpo
                      int64_t *offsets, int64_t *total_pairs, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     Prepared p;
-    int rc = launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, nullptr,
-                           nullptr, -1, s, p);
+    int rc = launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets,
+                           PairOut{nullptr, nullptr, 0}, -1, s, p);
     if (rc) return rc;
     return finish_search(p, offsets, total_pairs, s);
 }
 
 int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
                       const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
-                      int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity, void *stream) {
+                      int64_t *offsets, void *row, void *col, int32_t index_bytes, int64_t capacity,
+                      void *stream) {
     Prepared p;
+    PairOut out;
+    if (!pair_out(row, col, index_bytes, out)) return TCR_ERR_ARG;
     if (capacity < 0) {
         set_error("capacity must be >= 0");
         return TCR_ERR_ARG;
     }
-    return launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, row, col,
+    return launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, out,
                          capacity, (hipStream_t)stream, p);
+}
+
+struct tcr_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int tcr_graph_capture(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
+                      const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
+                      int64_t *offsets, void *row, void *col, int32_t index_bytes, int64_t capacity,
+                      void **graph, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    PairOut out;
+    if (!graph) {
+        set_error("graph handle missing");
+        return TCR_ERR_ARG;
+    }
+    if (!pair_out(row, col, index_bytes, out)) return TCR_ERR_ARG;
+    if (capacity < 0) {
+        set_error("capacity must be >= 0");
+        return TCR_ERR_ARG;
+    }
+    if (!s) {
+        set_error("graph capture needs a created stream, not the null stream");
+        return TCR_ERR_ARG;
+    }
+    *graph = nullptr;
+    TCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    Prepared p;
+    const int rc = launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, out,
+                                 capacity, s, p, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return TCR_ERR_HIP;
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        hipGraphDestroy(g);
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        return TCR_ERR_HIP;
+    }
+    *graph = new tcr_graph_s{g, x};
+    return TCR_OK;
+}
+
+int tcr_graph_launch(void *graph, void *stream) {
+    if (!graph) {
+        set_error("graph handle missing");
+        return TCR_ERR_ARG;
+    }
+    TCR_HIP(hipGraphLaunch(((tcr_graph_s *)graph)->exec, (hipStream_t)stream));
+    return TCR_OK;
+}
+
+int tcr_graph_destroy(void *graph) {
+    if (!graph) return TCR_OK;
+    tcr_graph_s *g = (tcr_graph_s *)graph;
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
+    return TCR_OK;
+}
+
+int tcr_set_tuning(const tcr_tuning *t) {
+    if (!t) {
+        g_tuning = Tuning();
+        return TCR_OK;
+    }
+    if (t->cell_workgroups < 0 || t->stencil_workgroups < 0 || t->cell_workgroups > (1 << 20) ||
+        t->stencil_workgroups > (1 << 20)) {
+        set_error("tuning values must be in [0, 2^20]");
+        return TCR_ERR_ARG;
+    }
+    g_tuning.cell_workgroups = t->cell_workgroups;
+    g_tuning.stencil_workgroups = t->stencil_workgroups;
+    return TCR_OK;
+}
+
+int tcr_get_tuning(tcr_tuning *t) {
+    if (!t) {
+        set_error("tuning pointer missing");
+        return TCR_ERR_ARG;
+    }
+    t->cell_workgroups = g_tuning.cell_workgroups;
+    t->stencil_workgroups = g_tuning.stencil_workgroups;
+    return TCR_OK;
 }
 
 int tcr_timing_enable(int32_t on) {
@@ -368,12 +476,18 @@ int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds, void *workspace
 
 int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
                     const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
-                    const int64_t *offsets, int64_t *row, int64_t *col, void *stream) {
+                    const int64_t *offsets, void *row, void *col, int32_t index_bytes, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     Prepared p;
+    PairOut out;
+    if (!pair_out(row, col, index_bytes, out)) return TCR_ERR_ARG;
     int rc = prepare(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false);
     if (rc) return rc;
-    return query_fill(p.b, p.w, offsets, row, col, INT64_MAX, s);
+    if (!offsets || !col) {
+        set_error("offsets or col missing");
+        return TCR_ERR_ARG;
+    }
+    return query_fill(p.b, p.w, offsets, out, INT64_MAX, s);
 }
 
 __global__ void k_export_sort_index(const int32_t *__restrict__ si, int64_t n, int64_t *__restrict__ out) {

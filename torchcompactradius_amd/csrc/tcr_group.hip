@@ -133,7 +133,7 @@ struct GroupArgs {
     const int64_t *total;        // offsets + nq: the fill runs only if *total <= capacity
     int64_t capacity;
     int64_t *soff;               // [nr] row offset of each sorted slot's query (fill)
-    int64_t *row, *col;          // fill outputs (row may be null)
+    PairOut out;                 // fill outputs
     int32_t *cursor;             // [nq] per-query write cursor across chunks (fill)
     int32_t *stray_list;         // [nq]
     uint32_t *stray_count;
@@ -547,13 +547,9 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
                 len += __popcll(hitm);
             }
             __asm__ volatile("" ::: "memory");
-            int64_t *colq = a.col + bcast(mbase, m);
-            int64_t *rowq = a.row ? a.row + bcast(mbase, m) : nullptr;
+            const int64_t rbase = bcast(mbase, m);
             const int64_t qloc = (int64_t)bcast(mloc, m);
-            for (int32_t k = lane; k < len; k += kWave) {
-                colq[k] = (int64_t)rowbuf[k];
-                if (rowq) rowq[k] = qloc;
-            }
+            for (int32_t k = lane; k < len; k += kWave) a.out.put(rbase + k, qloc, rowbuf[k]);
             __asm__ volatile("" ::: "memory");
         }
     }
@@ -617,8 +613,7 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
                 if (FILL) {
                     if ((hitm >> lane) & 1ull) {
                         const int64_t pos = base + __popcll(hitm & lanemask_lt());
-                        a.col[pos] = (int64_t)corig[r];
-                        if (a.row) a.row[pos] = qloc;
+                        a.out.put(pos, qloc, corig[r]);
                     }
                     base += __popcll(hitm);
                 }
@@ -735,9 +730,14 @@ __global__ void __launch_bounds__(256) k_sorted_offsets(const CloudDev *__restri
 
 // ------------------------------------------------------------- driver -----
 
+// Workgroups of the wavefront-per-cell kernels: a multiple of the 8 XCDs
+// (my_cells splits the cells by blockIdx % 8).
 static unsigned group_grid(const Batch &b) {
-    int64_t wgs = (b.nr_total + 3) / 4;   // upper bound on occupied cells / 4
-    if (wgs > 256 * 8) wgs = 256 * 8;
+    int64_t wgs = tuning().cell_workgroups;
+    if (wgs <= 0) {
+        wgs = (b.nr_total + 3) / 4;   // upper bound on occupied cells / 4
+        if (wgs > 256 * 8) wgs = 256 * 8;
+    }
     wgs = ((wgs + 7) / 8) * 8;
     return (unsigned)(wgs > 0 ? wgs : 8);
 }
@@ -766,11 +766,10 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
     return TCR_OK;
 }
 
-int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
-                   bool fill, bool list, hipStream_t s);
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
+                   bool list, hipStream_t s);
 
-static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
-                            int64_t capacity) {
+static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity) {
     GroupArgs ga;
     ga.desc = w.desc;
     ga.r_start = w.r_start;
@@ -791,8 +790,7 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, int
     ga.total = offsets ? offsets + b.nq_total : nullptr;
     ga.capacity = capacity;
     ga.soff = w.soff;
-    ga.row = row;
-    ga.col = col;
+    ga.out = out;
     ga.cursor = w.cursor;
     ga.stray_list = w.stray_list;
     ga.stray_count = w.misc + kMiscStrays;
@@ -804,8 +802,8 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, int
 // Count or fill pass over the batch: wavefront-per-cell kernels for the
 // self-search clouds (their strays collected by the count pass), the
 // per-query kernel for every other cloud and for the strays.
-static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col,
-                      int64_t capacity, bool fill, hipStream_t s) {
+static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
+                      hipStream_t s) {
     if (b.nq_total == 0) return TCR_OK;
     bool any_group = false, any_other = false;
     for (const CloudDev &d : b.dev) {
@@ -814,11 +812,12 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
         else any_other = true;
     }
     if (any_group) {
-        const GroupArgs ga = group_args(b, w, offsets, row, col, capacity);
+        const GroupArgs ga = group_args(b, w, offsets, out, capacity);
         if (!fill) {
             // misc[kMiscStrays] and misc[kMiscBig] start at zero: the
             // header upload clears all of misc
-            hipLaunchKernelGGL(k_stencil, dim3(2048), dim3(256), 0, s, ga);
+            const int32_t sw = tuning().stencil_workgroups;
+            hipLaunchKernelGGL(k_stencil, dim3(sw > 0 ? (unsigned)sw : 2048u), dim3(256), 0, s, ga);
         } else {
             hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((b.nr_total + 255) / 256)), dim3(256), 0, s,
                                (const CloudDev *)w.desc, b.B, (const int64_t *)w.r_start,
@@ -827,19 +826,18 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, int64_t *
         }
         int rc = fill ? launch_modes<true>(b, ga, s) : launch_modes<false>(b, ga, s);
         if (rc) return rc;
-        rc = launch_generic(b, w, offsets, row, col, capacity, fill, true, s);
+        rc = launch_generic(b, w, offsets, out, capacity, fill, true, s);
         if (rc) return rc;
     }
-    if (any_other) return launch_generic(b, w, offsets, row, col, capacity, fill, false, s);
+    if (any_other) return launch_generic(b, w, offsets, out, capacity, fill, false, s);
     return TCR_OK;
 }
 
 int query_count(const Batch &b, Work &w, hipStream_t s) {
-    return query_pass(b, w, nullptr, nullptr, nullptr, -1, false, s);
+    return query_pass(b, w, nullptr, PairOut{nullptr, nullptr, 0}, -1, false, s);
 }
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
-               hipStream_t s) {
-    return query_pass(b, w, offsets, row, col, capacity, true, s);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s) {
+    return query_pass(b, w, offsets, out, capacity, true, s);
 }
 
 }  // namespace tcr

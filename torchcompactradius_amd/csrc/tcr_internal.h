@@ -161,14 +161,36 @@ int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64
                    hipStream_t s);
 size_t scan_partials_needed(int64_t n);
 
+// Launch-shape parameters (tcr_set_tuning); 0 = automatic.
+struct Tuning {
+    int32_t cell_workgroups = 0;
+    int32_t stencil_workgroups = 0;
+};
+Tuning &tuning();
+
+// Pair outputs of the fill: row (may be null) and col, with int64 entries
+// (the reference's dtype) or int32 entries.
+struct PairOut {
+    void *row, *col;
+    int32_t idx32;
+    __device__ __forceinline__ void put(int64_t pos, int64_t r, int32_t c) const {
+        if (idx32) {
+            ((int32_t *)col)[pos] = c;
+            if (row) ((int32_t *)row)[pos] = (int32_t)r;
+        } else {
+            ((int64_t *)col)[pos] = c;
+            if (row) ((int64_t *)row)[pos] = r;
+        }
+    }
+};
+
 // Structure build (hMax, grid, keys, sort, cells, lin table).  Expects the
 // header block (descriptors, prefixes, zeroed misc) uploaded.
 int build_structure(const Batch &b, Work &w, hipStream_t s);
 // Count pass (counts[] and the fill's masks); fill pass into offsets,
 // skipped on the device when offsets[nq_total] > capacity.
 int query_count(const Batch &b, Work &w, hipStream_t s);
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
-               hipStream_t s);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s);
 // Reference-layout bucket tables of every cloud (for export).
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted

@@ -156,7 +156,7 @@ struct GenericArgs {
     const int64_t *offsets;
     const int64_t *total;       // offsets + nq: the fill runs only if *total <= capacity
     int64_t capacity;
-    int64_t *row, *col;
+    PairOut out;
     const int32_t *list;        // stray list (list mode)
     const uint32_t *list_count;
 };
@@ -175,10 +175,7 @@ __device__ __forceinline__ void query_one(const GenericArgs &a, const CloudDev &
     } else {
         const int64_t base = a.offsets[g];
         const int32_t *si = a.sort_index + rs;
-        traverse<DIM>(gm, lk, cands, xi, hi, [&](int64_t k, int64_t j) {
-            a.col[base + k] = (int64_t)si[j];
-            if (a.row) a.row[base + k] = local;
-        });
+        traverse<DIM>(gm, lk, cands, xi, hi, [&](int64_t k, int64_t j) { a.out.put(base + k, local, si[j]); });
     }
 }
 
@@ -220,8 +217,8 @@ __global__ void __launch_bounds__(kQBlock) k_query_list(GenericArgs a) {
         generic_query<FILL>(a, a.list[i]);
 }
 
-int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row, int64_t *col, int64_t capacity,
-                   bool fill, bool list, hipStream_t s) {
+int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
+                   bool list, hipStream_t s) {
     GenericArgs a;
     a.desc = w.desc;
     a.B = b.B;
@@ -237,8 +234,7 @@ int launch_generic(const Batch &b, Work &w, const int64_t *offsets, int64_t *row
     a.offsets = offsets;
     a.total = offsets ? offsets + b.nq_total : nullptr;
     a.capacity = capacity;
-    a.row = row;
-    a.col = col;
+    a.out = out;
     a.list = w.stray_list;
     a.list_count = w.misc + kMiscStrays;
     if (list) {

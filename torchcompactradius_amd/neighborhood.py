@@ -40,6 +40,8 @@ def radiusSearch(
         returnStructure: bool = False,
         useDenseMLM: bool = False,
         hashMapLengthAlgorithm: str = "primes",
+        *,
+        indexDtype: torch.dtype = torch.int64,
 ) -> Union[SparseCOO, SparseCSR, Tuple[Union[SparseCOO, SparseCSR], CompactHashMap]]:
     """Fixed-radius neighbour search (neighborhood.py:136-309).
 
@@ -52,6 +54,8 @@ def radiusSearch(
     Only ``algorithm='compact'`` is computed by this package; every search
     runs on the GPU.  ``supportOverride`` maps a scalar support onto every
     point (the reference's fixed-support compact variant is not callable).
+    ``indexDtype`` (keyword only, an extension) selects int64 row/col
+    indices -- the reference's -- or int32 ones.
     """
     with torch.no_grad():
         assert algorithm in ALGORITHMS, f"algorithm = {algorithm} not supported"
@@ -95,7 +99,7 @@ def radiusSearch(
                   f"dim={dim} mode={mode} periodic_mask={pmask} hashMapLength={hashMapLength} format={format}")
         spec = CloudSpec(qpos=qpos, qsup=qsup, rpos=rpos, rsup=rsup, domain_min=mn, domain_max=mx,
                          periodic=pmask, mode=mode, hash_map_length=int(hashMapLength))
-        res = run_search([spec], want_rows=(format == "coo"))
+        res = run_search([spec], want_rows=(format == "coo"), index_dtype=indexDtype)
         nq = qpos.shape[0]
         nr = nq if rpos is None else rpos.shape[0]
         if format == "coo":

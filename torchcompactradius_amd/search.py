@@ -1,6 +1,8 @@
 """Host driver of the compact search: validates tensors, fills ``tcr_cloud``
-descriptors and runs count -> (one sync for the pair total) -> fill through
-libtcr_hip.so on the current HIP stream.
+descriptors and runs build -> count -> scan -> fill through libtcr_hip.so on
+the current HIP stream -- back to back into outputs sized from the previous
+search of the same batch shape, or as count, one sync for the pair total,
+then fill -- optionally replayed from a captured HIP graph.
 
 Reference counterpart: neighborSearch_cpp + searchNeighbors_cpp
 (src/torchCompactRadius/compactHashing/cppSearch.py:12-98,179-244).
@@ -142,16 +144,8 @@ def _remember(key, total: int) -> None:
     _pair_hint[key] = total
 
 
-def run_search(specs: Sequence[CloudSpec], want_rows: bool, *, qpos_cat=None, qsup_cat=None,
-               rpos_cat=None, rsup_cat=None, timers: Optional[dict] = None) -> SearchResult:
-    """Search a batch of clouds; inputs are concatenated into flat device
-    buffers unless already given (``*_cat``) with offsets implied by order.
-    ``timers`` (optional) receives torch.cuda.Event pairs recorded on the
-    search stream around the count and fill launches."""
-    L = _lib.load()
-    if len(specs) == 0:
-        raise ValueError("empty batch")
-    device = specs[0].qpos.device
+def _concat(specs: Sequence[CloudSpec], device):
+    """Flat query/reference buffers of a batch and each cloud's offsets."""
     q_offs, r_offs = [], []
     qo = qf = ro = rf = 0
     for s in specs:
@@ -164,74 +158,155 @@ def run_search(specs: Sequence[CloudSpec], want_rows: bool, *, qpos_cat=None, qs
             rf += s.rpos.numel()
         else:
             r_offs.append((0, 0))
-    if qpos_cat is None:
-        if len(specs) == 1:
-            qpos_cat = specs[0].qpos.reshape(-1)
-            qsup_cat = specs[0].qsup
-            rpos_cat = None if specs[0].rpos is None else specs[0].rpos.reshape(-1)
-            rsup_cat = specs[0].rsup
+    if len(specs) == 1:
+        s = specs[0]
+        cat = (s.qpos.reshape(-1), s.qsup, None if s.rpos is None else s.rpos.reshape(-1), s.rsup)
+    else:
+        qpos_cat = torch.cat([s.qpos.reshape(-1) for s in specs])
+        qs = [s.qsup for s in specs]
+        qsup_cat = torch.cat([q if q is not None else torch.zeros(s.qpos.shape[0], device=device)
+                              for q, s in zip(qs, specs)]) if any(q is not None for q in qs) else None
+        rl = [s for s in specs if s.rpos is not None]
+        rpos_cat = torch.cat([s.rpos.reshape(-1) for s in rl]) if rl else None
+        rsl = [s.rsup for s in rl]
+        rsup_cat = torch.cat([r if r is not None else torch.zeros(s.rpos.shape[0], device=device)
+                              for r, s in zip(rsl, rl)]) if any(r is not None for r in rsl) else None
+        cat = (qpos_cat, qsup_cat, rpos_cat, rsup_cat)
+    return cat, q_offs, r_offs, qo, ro
+
+
+class SearchPlan:
+    """A batch of clouds bound to flat device buffers and a workspace: the
+    unit that one search launch, a captured graph and the serving loop run.
+
+    The buffers (``qpos``, ``qsup``, ``rpos``, ``rsup``) are concatenations of
+    the clouds' tensors in order (or the caller's own tensors for one cloud);
+    rewriting them in place and running the plan again searches the new
+    values.  ``index_dtype`` is torch.int64 (the reference's) or torch.int32.
+    """
+
+    def __init__(self, specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.int64):
+        if len(specs) == 0:
+            raise ValueError("empty batch")
+        if index_dtype not in (torch.int64, torch.int32):
+            raise ValueError("index_dtype must be torch.int64 or torch.int32")
+        self.L = _lib.load()
+        self.specs = list(specs)
+        self.want_rows = want_rows
+        self.index_dtype = index_dtype
+        self.index_bytes = 8 if index_dtype == torch.int64 else 4
+        self.device = specs[0].qpos.device
+        (self.qpos, self.qsup, self.rpos, self.rsup), q_offs, r_offs, self.nq, self.nr = _concat(specs, self.device)
+        self.clouds = _cloud_array(specs, q_offs, r_offs)
+        self.n = len(specs)
+        self.ws_bytes = self.L.tcr_search_workspace_size(self.clouds, self.n)
+        if self.ws_bytes == 0:
+            check(_lib.TCR_ERR_ARG)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
+        self.offsets = torch.empty(self.nq + 1, dtype=torch.int64, device=self.device)
+        self.key = (self.n, self.nq, self.nr, tuple((c.dim, c.mode) for c in self.clouds[:8]))
+        self._graph = None
+        self._graph_out = None
+        self.tuning = None          # (cell_workgroups, stencil_workgroups) or None: automatic
+
+    def _install_tuning(self):
+        from .autotune import install
+        install(self.tuning)
+
+    def _args(self):
+        return (self.clouds, self.n, ptr(self.qpos), ptr(self.qsup), ptr(self.rpos), ptr(self.rsup),
+                ptr(self.ws), self.ws_bytes)
+
+    def _alloc(self, n: int):
+        col = torch.empty(n, dtype=self.index_dtype, device=self.device)
+        row = torch.empty(n, dtype=self.index_dtype, device=self.device) if self.want_rows else None
+        return row, col
+
+    def _finish(self, stream) -> int:
+        total = ctypes.c_int64(0)
+        check(self.L.tcr_search_finish(self.clouds, self.n, ptr(self.ws), self.ws_bytes, ptr(self.offsets),
+                                       ctypes.byref(total), stream))
+        return int(total.value)
+
+    def _fill_exact(self, P: int, stream):
+        row, col = self._alloc(P)
+        if P > 0:  # nothing to write otherwise (and an empty tensor has no data pointer)
+            check(self.L.tcr_search_fill(*self._args(), ptr(self.offsets), ptr(row), ptr(col), self.index_bytes,
+                                         stream))
+        return row, col
+
+    def run(self, capacity: Optional[int] = None) -> "SearchResult":
+        """One search.  With a capacity guess (default: from the last search
+        of this batch shape) count, scan and fill run back to back; a guess
+        that is too small costs one extra fill launch."""
+        stream = _stream()
+        self._install_tuning()
+        if capacity is None:
+            hint = _pair_hint.get(self.key)
+            capacity = None if hint is None else hint + hint // 32 + 256
+        if capacity is not None:
+            row_buf, col_buf = self._alloc(capacity)
+            check(self.L.tcr_search_launch(*self._args(), ptr(self.offsets), ptr(row_buf), ptr(col_buf),
+                                           self.index_bytes, capacity, stream))
+            P = self._finish(stream)
+            _remember(self.key, P)
+            if P <= capacity:
+                return SearchResult(self.offsets, None if row_buf is None else row_buf[:P], col_buf[:P], P,
+                                    self.ws, self.clouds)
         else:
-            qpos_cat = torch.cat([s.qpos.reshape(-1) for s in specs])
-            qs = [s.qsup for s in specs]
-            qsup_cat = torch.cat([q if q is not None else torch.zeros(s.qpos.shape[0], device=device)
-                                  for q, s in zip(qs, specs)]) if any(q is not None for q in qs) else None
-            rl = [s for s in specs if s.rpos is not None]
-            rpos_cat = torch.cat([s.rpos.reshape(-1) for s in rl]) if rl else None
-            rsl = [s.rsup for s in rl]
-            rsup_cat = torch.cat([r if r is not None else torch.zeros(s.rpos.shape[0], device=device)
-                                  for r, s in zip(rsl, rl)]) if any(r is not None for r in rsl) else None
-    clouds = _cloud_array(specs, q_offs, r_offs)
-    n = len(specs)
-    ws_bytes = L.tcr_search_workspace_size(clouds, n)
-    if ws_bytes == 0:
-        check(_lib.TCR_ERR_ARG)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
-    offsets = torch.empty(qo + 1, dtype=torch.int64, device=device)
-    total = ctypes.c_int64(0)
-    stream = _stream()
+            total = ctypes.c_int64(0)
+            check(self.L.tcr_search_count(*self._args(), ptr(self.offsets), ctypes.byref(total), stream))
+            P = int(total.value)
+            _remember(self.key, P)
+        row, col = self._fill_exact(P, stream)
+        return SearchResult(self.offsets, row, col, P, self.ws, self.clouds)
 
-    def ev():
-        e = torch.cuda.Event(enable_timing=True)
-        e.record()
-        return e
+    def capture(self, capacity: int) -> None:
+        """Capture the whole search into a HIP graph with outputs of
+        ``capacity`` pairs (replayed by ``replay``)."""
+        self.release_graph()
+        row, col = self._alloc(capacity)
+        self._install_tuning()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        handle = ctypes.c_void_p(0)
+        check(self.L.tcr_graph_capture(*self._args(), ptr(self.offsets), ptr(row), ptr(col), self.index_bytes,
+                                       capacity, ctypes.byref(handle), side.cuda_stream))
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        self._graph = handle
+        self._graph_out = (row, col, capacity)
 
-    key = (n, qo, ro)
-    hint = _pair_hint.get(key)
-    if timers is None and hint is not None:
-        # speculative outputs sized from the last search of this shape: the
-        # whole chain runs without a host round trip between count and fill
-        cap = hint + hint // 32 + 256
-        col_buf = torch.empty(cap, dtype=torch.int64, device=device)
-        row_buf = torch.empty(cap, dtype=torch.int64, device=device) if want_rows else None
-        check(L.tcr_search_launch(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
-                                  ptr(ws), ws_bytes, ptr(offsets), ptr(row_buf), ptr(col_buf), cap, stream))
-        check(L.tcr_search_finish(clouds, n, ptr(ws), ws_bytes, ptr(offsets), ctypes.byref(total), stream))
-        P = int(total.value)
-        _remember(key, P)
+    def replay(self) -> "SearchResult":
+        """Replay the captured graph on the current stream.  The outputs are
+        the captured buffers (views of their first P entries): consume or copy
+        them before the next replay."""
+        if self._graph is None:
+            raise RuntimeError("no captured graph: call capture() first")
+        stream = _stream()
+        check(self.L.tcr_graph_launch(self._graph, stream))
+        P = self._finish(stream)
+        row, col, cap = self._graph_out
         if P <= cap:
-            return SearchResult(offsets, None if row_buf is None else row_buf[:P], col_buf[:P], P, ws, clouds)
-        col = torch.empty(P, dtype=torch.int64, device=device)
-        row = torch.empty(P, dtype=torch.int64, device=device) if want_rows else None
-        check(L.tcr_search_fill(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
-                                ptr(ws), ws_bytes, ptr(offsets), ptr(row), ptr(col), stream))
-        return SearchResult(offsets, row, col, P, ws, clouds)
+            return SearchResult(self.offsets, None if row is None else row[:P], col[:P], P, self.ws, self.clouds)
+        row, col = self._fill_exact(P, stream)
+        return SearchResult(self.offsets, row, col, P, self.ws, self.clouds)
 
-    t0 = ev() if timers is not None else None
-    check(L.tcr_search_count(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
-                             ptr(ws), ws_bytes, ptr(offsets), ctypes.byref(total), stream))
-    t1 = ev() if timers is not None else None
-    P = int(total.value)
-    _remember(key, P)
-    col = torch.empty(P, dtype=torch.int64, device=device)
-    row = torch.empty(P, dtype=torch.int64, device=device) if want_rows else None
-    t2 = ev() if timers is not None else None
-    check(L.tcr_search_fill(clouds, n, ptr(qpos_cat), ptr(qsup_cat), ptr(rpos_cat), ptr(rsup_cat),
-                            ptr(ws), ws_bytes, ptr(offsets), ptr(row), ptr(col), stream))
-    if timers is not None:
-        t3 = ev()
-        timers.setdefault("count", []).append((t0, t1))
-        timers.setdefault("fill", []).append((t2, t3))
-    return SearchResult(offsets, row, col, P, ws, clouds)
+    def release_graph(self) -> None:
+        if self._graph is not None:
+            self.L.tcr_graph_destroy(self._graph)
+            self._graph = None
+            self._graph_out = None
+
+    def __del__(self):
+        try:
+            self.release_graph()
+        except Exception:
+            pass
+
+
+def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.int64) -> "SearchResult":
+    """Search a batch of clouds in one launch chain (see SearchPlan)."""
+    return SearchPlan(specs, want_rows, index_dtype).run()
 
 
 def export_structure(res: SearchResult, cloud: int, spec: CloudSpec) -> CompactHashMap:

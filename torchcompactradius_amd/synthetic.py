@@ -73,3 +73,26 @@ def periodic_3d(n: int = 100, seed: int = 2, neighbors: float = 50.0) -> Workloa
     return Workload(f"periodic-3d-{n**3}", pos, np.full(pos.shape[0], h, np.float32),
                     np.full(dim, -1, np.float32), np.full(dim, 1, np.float32), [True] * dim,
                     "gather", 1 << 20, "csr", meta={"dx": dx, "h": float(h), "n": n})
+
+
+def batched_clouds(seed: int = 4, n_clouds: int = 256, n_min: float = 1e3, n_max: float = 2e5,
+                   scale: float = 1.0) -> List[Workload]:
+    """Independent clouds for one batched launch: sizes log-uniform in
+    [n_min, n_max] (times ``scale``), alternating 2-D / 3-D jittered grids in
+    [-1,1]^d (jitter +-0.1 dx), per-cloud support for U{20..60} neighbours,
+    per-dimension periodicity ~ Bernoulli(0.5), gather, CSR."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_clouds):
+        dim = 2 if k % 2 == 0 else 3
+        N = math.exp(rng.uniform(math.log(n_min), math.log(n_max))) * scale
+        n = max(2, int(round(N ** (1.0 / dim))))
+        dx = 2.0 / n
+        pos = jittered_grid(n, dim, -1.0, 1.0, 0.1, rng)
+        neighbors = int(rng.integers(20, 61))
+        h = np.float32(support_for(neighbors, dx, dim))
+        periodic = [bool(x) for x in (rng.random(dim) < 0.5)]
+        out.append(Workload(f"batched-{k}-{dim}d-{pos.shape[0]}", pos, np.full(pos.shape[0], h, np.float32),
+                            np.full(dim, -1, np.float32), np.full(dim, 1, np.float32), periodic,
+                            "gather", 1 << 16, "csr", meta={"dx": dx, "h": float(h), "neighbors": neighbors}))
+    return out
