@@ -153,6 +153,47 @@ def bench_batched(torch, reps: int = 5, scale: float = 1.0):
             "speedup_vs_plain": t_plain / (res["int64"]["ms"] / 1e3)}
 
 
+def bench_served(torch, n_requests: int = 512, rates=(100000.0, 1500.0), runs: int = 64):
+    """Served stream: Poisson arrivals of 3-D search requests from `runs`
+    simulated SPH runs (sizes log-uniform 1e3..1e6, every step displaced by
+    at most 0.1 h), admitted under a 200 GB HBM budget.  Continuous batching
+    (batched launches on 2 streams, resident per-run plans) against the same
+    stream served by per-request plain launches, one at a time; every
+    request's result must be identical in both (fingerprints compared)."""
+    from torchcompactradius_amd.serve import (GpuBackend, Scheduler, latency_stats, make_simulations,
+                                              poisson_requests)
+    out = {"runs": runs, "requests": n_requests, "budget_GB": 200}
+    warm = make_simulations(torch, n_runs=8, n_min=1e3, n_max=1e5, seed=99)
+    for batched in (True, False):
+        be = GpuBackend(torch, warm, slots=2, batched=batched)
+        Scheduler(be, max_batch=64 if batched else 1).run(
+            poisson_requests(32, 20000.0, [s.pos.shape[0] for s in warm], 98, [s.last_pairs for s in warm]))
+    torch.cuda.synchronize()
+    for rate in rates:
+        rec = {}
+        digests = {}
+        for name, batched in (("batched", True), ("plain", False)):
+            sims = make_simulations(torch, n_runs=runs, seed=5)
+            npts = [s.pos.shape[0] for s in sims]
+            reqs = poisson_requests(n_requests, rate, npts, 5, [s.last_pairs for s in sims])
+            be = GpuBackend(torch, sims, slots=2, batched=batched)
+            sch = Scheduler(be, budget_bytes=200e9, max_batch=64 if batched else 1)
+            done = sch.run(reqs)
+            torch.cuda.synchronize()
+            wall = max(r.done for r in done) - min(r.arrival for r in done)
+            st = latency_stats(done)
+            st.update({"pairs": be.pairs, "pairs_per_s": be.pairs / wall, "wall_s": wall,
+                       "points": int(sum(r.n for r in done)),
+                       "mean_batch": float(np.mean(sch.batches)), "peak_in_use_GB": sch.peak_in_use / 1e9})
+            rec[name] = st
+            digests[name] = {r.rid: be.digest(r) for r in done}   # after the timed stream
+            del sims, be, done
+        rec["bit_exact_alone_vs_mix"] = digests["batched"] == digests["plain"]
+        rec["speedup_pairs_per_s"] = rec["batched"]["pairs_per_s"] / rec["plain"]["pairs_per_s"]
+        out[f"rate_{int(rate)}_per_s"] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +202,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="grid points per dimension (100 -> 1M points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (256-cloud) comparison")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -241,6 +283,8 @@ def main():
     }
     if not args.no_batched:
         line["batched"] = bench_batched(torch)
+    if not args.no_served:
+        line["served"] = bench_served(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(line), flush=True)
