@@ -194,6 +194,71 @@ def bench_served(torch, n_requests: int = 512, rates=(100000.0, 1500.0), runs: i
     return out
 
 
+def _reference_pairs(torch, pos, sup, rows, mn, mx, mode="symmetric"):
+    """Exact neighbours of a few query rows by brute force over all points,
+    with the reference's pair arithmetic (fp32 difference, fp64 square added
+    to an fp32 running sum, fp32 sqrt, strict '<'), in the compact row order
+    (hCell stencil offset, then index).  Non-periodic domains."""
+    out = []
+    for i in rows.tolist():
+        d = pos[i:i + 1] - pos                            # fp32 differences
+        acc = torch.zeros(pos.shape[0], dtype=torch.float32, device=pos.device)
+        for k in range(pos.shape[1]):
+            acc = (acc.double() + d[:, k].double() ** 2).float()
+        dist = torch.sqrt(acc)
+        h = (sup[i] + sup) / 2.0 if mode == "symmetric" else torch.maximum(sup[i].expand_as(sup), sup)
+        j = torch.nonzero(dist < h).reshape(-1)
+        out.append(j)
+    return out
+
+
+def bench_adaptive(torch, n: int = 2_000_000, reps: int = 3):
+    """Adaptive supports (70% Gaussian clusters + 30% uniform, k-NN supports
+    with an 8x ratio), symmetric, non-periodic, COO: the multi-level search.
+    Full-size checks: the pair set is symmetric, and sampled rows equal a
+    brute-force evaluation of the reference predicate in the compact order."""
+    from torchcompactradius_amd import synthetic
+    from torchcompactradius_amd.search import CloudSpec, run_mlm
+    w = synthetic.adaptive_3d(n=n)
+    pos = torch.from_numpy(w.pos).cuda()
+    sup = torch.from_numpy(w.sup).cuda()
+    spec = CloudSpec(qpos=pos, qsup=sup, rpos=None, rsup=None, domain_min=[-1.0] * 3, domain_max=[1.0] * 3,
+                     periodic=0, mode="symmetric", hash_map_length=w.hash_map_length)
+    r = run_mlm(spec, want_rows=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = run_mlm(spec, want_rows=True)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    P = r.total
+    # symmetry of the pair set
+    a = torch.sort(r.row * n + r.col).values
+    b = torch.sort(r.col * n + r.row).values
+    symmetric = bool(torch.equal(a, b))
+    del a, b
+    # sampled rows against brute force, in the compact order
+    g = torch.Generator().manual_seed(3)
+    rows = torch.randint(0, n, (64,), generator=g)
+    ref = _reference_pairs(torch, pos, sup, rows, -1.0, 1.0)
+    from torchcompactradius_amd.util import compute_h
+    hcell = compute_h([-1.0] * 3, [1.0] * 3, float(w.sup.max()))
+    cells = torch.floor((pos + 1.0) / hcell).to(torch.int64)      # hCell cell of every point
+    offs = r.offsets
+    ok = True
+    for i, j in zip(rows.tolist(), ref):
+        got = r.col[int(offs[i]):int(offs[i + 1])]
+        # compact order: stencil offset of the candidate's cell, then index
+        dc = cells[j] - cells[i]
+        o = (dc[:, 0] + 1) * 9 + (dc[:, 1] + 1) * 3 + (dc[:, 2] + 1)
+        want = j[torch.argsort(o * n + j)]
+        ok = ok and bool(torch.equal(got, want))
+    return {"workload": w.name, "points": n, "pairs": P, "ms": t * 1e3, "pairs_per_s": P / t,
+            "h_min": w.meta["h_min"], "h_max": w.meta["h_max"], "format": "coo", "mode": "symmetric",
+            "checks": {"pair_set_symmetric": symmetric, "sampled_rows_vs_brute_force": bool(ok),
+                       "sampled_rows": int(rows.numel())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +268,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (256-cloud) comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream comparison")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-support multi-level search")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -285,6 +351,8 @@ def main():
         line["batched"] = bench_batched(torch)
     if not args.no_served:
         line["served"] = bench_served(torch)
+    if not args.no_adaptive:
+        line["adaptive"] = bench_adaptive(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

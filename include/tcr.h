@@ -40,6 +40,8 @@ extern "C" {
 #define TCR_ERR_HIP (-2)      /* a HIP runtime call failed                      */
 #define TCR_ERR_DOMAIN (-3)   /* degenerate grid (e.g. zero cells in a periodic dim) */
 #define TCR_ERR_SPACE (-4)    /* workspace or output smaller than required      */
+#define TCR_ERR_UNSUPPORTED (-5) /* inputs the multi-level search does not take; the
+                                    single-level search gives the same output    */
 
 /* Support modes, same meaning as the reference's mode strings
  * (cpp/common.h:68-70, cpp/neighborhoodDynamic_cpu.cpp:77-88). */
@@ -141,6 +143,30 @@ int tcr_get_tuning(tcr_tuning *tuning);
 int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds,
                       void *workspace, size_t workspace_bytes,
                       const int64_t *offsets, int64_t *total_pairs, void *stream);
+
+/* Multi-level search of one cloud for variable supports: the same pairs in
+ * the same order as tcr_search_count/_fill, with reference points binned
+ * into up to max_levels (1..4) nested grids of cell size hCell * 2^-l by
+ * their support.  Replaces the reference's algorithm='mlm' path
+ * (py/neighborhood.py:293, py/multiLevelMemory/wrapper.py:145-308,
+ * cpp/multiLevelMemory/countNeighborsHashmap.h:99-124,
+ * buildNeighborhoodHashmap.h:103-128), whose output is defined as the compact
+ * path's.  Returns TCR_ERR_UNSUPPORTED for inputs it does not take (gather
+ * mode, search_range != 1, non-finite or negative supports, points outside
+ * the grid, fewer than 3 cells in a dimension, grids beyond 2^29 cells): run
+ * the single-level search then.  tcr_mlm_count synchronises twice and writes
+ * offsets[nq + 1], the pair total and the level count to pass to
+ * tcr_mlm_fill; scratch holds total_pairs uint64 and may be col itself when
+ * index_bytes is 8. */
+size_t tcr_mlm_workspace_size(const tcr_cloud *cloud);
+int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels,
+                  const float *qpos, const float *qsup, const float *rpos, const float *rsup,
+                  void *workspace, size_t workspace_bytes, int64_t *offsets,
+                  int64_t *total_pairs, int32_t *levels_used, void *stream);
+int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels,
+                 const float *qpos, const float *qsup, const float *rpos, const float *rsup,
+                 void *workspace, size_t workspace_bytes, const int64_t *offsets,
+                 void *row, void *col, int32_t index_bytes, uint64_t *scratch, void *stream);
 
 /* Copy one cloud's structure out of the workspace in the reference layout:
  * sort_index int64[nr], sorted_pos fp32[nr, dim], sorted_sup fp32[nr] (NULL to

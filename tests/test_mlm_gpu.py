@@ -1,0 +1,121 @@
+"""GPU parity of the multi-level search (algorithm='mlm'): the same pairs in
+the same order as the compact search and the oracle, for variable supports
+in every mode that bounds the radius by the reference supports, 1-3 dims,
+periodic and not; inputs it does not take fall back to the compact search."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch, synthetic
+from torchcompactradius_amd._lib import Unsupported
+from torchcompactradius_amd.search import CloudSpec, run_mlm
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _search(pos, sup, mn, mx, per, mode, algorithm, fmt="coo", rpos=None, rsup=None, index_dtype=torch.int64):
+    dom = DomainDescription(torch.tensor(mn, dtype=torch.float32), torch.tensor(mx, dtype=torch.float32),
+                            torch.tensor(per), len(mn))
+    ref = None if rpos is None else PointCloud(_t(rpos), _t(rsup))
+    out = radiusSearch(PointCloud(_t(pos), _t(sup)), ref, mode=mode, domain=dom, hashMapLength=4096,
+                       algorithm=algorithm, format=fmt, indexDtype=index_dtype)
+    torch.cuda.synchronize()
+    return out
+
+
+def _spec(pos, sup, mn, mx, per, mode):
+    return CloudSpec(qpos=_t(pos), qsup=_t(sup), rpos=None, rsup=None, domain_min=list(mn), domain_max=list(mx),
+                     periodic=sum(1 << d for d in range(len(per)) if per[d]), mode=mode, hash_map_length=4096)
+
+
+def test_adaptive_cloud_matches_oracle_and_compact():
+    w = synthetic.adaptive_3d(n=30000, seed=3)
+    print("case: adaptive 30000 symmetric", flush=True)
+    spec = _spec(w.pos, w.sup, w.domain_min, w.domain_max, w.periodic, "symmetric")
+    r = run_mlm(spec, want_rows=True)          # raises if it fell back
+    torch.cuda.synchronize()
+    o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                             mode="symmetric", hash_map_length=4096)
+    np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+    np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+    c = _search(w.pos, w.sup, w.domain_min, w.domain_max, w.periodic, "symmetric", "compact")
+    assert torch.equal(c.row, r.row) and torch.equal(c.col, r.col)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("pmask", [0, 5])
+def test_modes_dims_periodicity(dim, mode, pmask):
+    rng = np.random.default_rng(700 + dim * 10 + pmask + len(mode))
+    n = 4000
+    per = [bool((pmask >> d) & 1) for d in range(dim)]
+    pos = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    base = {1: 0.004, 2: 0.03, 3: 0.05}[dim]
+    sup = (base * np.exp2(rng.uniform(0, 3, n))).astype(np.float32)   # ratio up to 8
+    mn, mx = [-1.0] * dim, [1.0] * dim
+    print(f"case: mlm dim={dim} mode={mode} pmask={pmask}", flush=True)
+    r = run_mlm(_spec(pos, sup, mn, mx, per, mode), want_rows=True)
+    torch.cuda.synchronize()
+    o = oracle.radius_search(pos, sup, domain_min=mn, domain_max=mx, periodic=per, mode=mode, hash_map_length=4096)
+    np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+    np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+
+
+def test_int32_csr_and_levels_capped():
+    w = synthetic.adaptive_3d(n=20000, seed=9, ratio=8.0)
+    print("case: mlm int32 csr", flush=True)
+    for levels in (1, 2, 4):
+        r = run_mlm(_spec(w.pos, w.sup, w.domain_min, w.domain_max, w.periodic, "symmetric"), want_rows=False,
+                    index_dtype=torch.int32, max_levels=levels)
+        torch.cuda.synchronize()
+        o = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max,
+                                 periodic=w.periodic, mode="symmetric", hash_map_length=4096, want_rows=False)
+        assert r.col.dtype == torch.int32
+        np.testing.assert_array_equal(r.col.cpu().numpy().astype(np.int64), o.col)
+        np.testing.assert_array_equal(r.offsets.cpu().numpy(), o.indptr.astype(np.int64))
+
+
+def test_unsupported_inputs_fall_back_to_compact():
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-1, 1, (3000, 3)).astype(np.float32)
+    sup = rng.uniform(0.05, 0.3, 3000).astype(np.float32)
+    mn, mx = [-1.0] * 3, [1.0] * 3
+    print("case: mlm gather falls back", flush=True)
+    with pytest.raises(Unsupported):
+        run_mlm(_spec(pos, sup, mn, mx, [False] * 3, "gather"), want_rows=True)
+    out_pos = pos.copy()
+    out_pos[:10] *= 1.5                       # outside [-1, 1]: outside the grid
+    with pytest.raises(Unsupported):
+        run_mlm(_spec(out_pos, sup, mn, mx, [False] * 3, "symmetric"), want_rows=True)
+    for p, mode in ((pos, "gather"), (out_pos, "symmetric")):
+        a = _search(p, sup, mn, mx, [False] * 3, mode, "mlm")
+        o = oracle.radius_search(p, sup, domain_min=mn, domain_max=mx, periodic=[False] * 3, mode=mode,
+                                 hash_map_length=4096)
+        np.testing.assert_array_equal(a.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(a.col.cpu().numpy(), o.col)
+
+
+def test_non_self_search():
+    rng = np.random.default_rng(12)
+    q = rng.uniform(-1, 1, (2500, 3)).astype(np.float32)
+    r = rng.uniform(-1, 1, (3500, 3)).astype(np.float32)
+    hq = (0.05 * np.exp2(rng.uniform(0, 3, 2500))).astype(np.float32)
+    hr = (0.05 * np.exp2(rng.uniform(0, 3, 3500))).astype(np.float32)
+    mn, mx = [-1.0] * 3, [1.0] * 3
+    for mode in ("symmetric", "scatter"):
+        print(f"case: mlm non-self {mode}", flush=True)
+        spec = CloudSpec(qpos=_t(q), qsup=_t(hq), rpos=_t(r), rsup=_t(hr), domain_min=mn, domain_max=mx,
+                         periodic=5, mode=mode, hash_map_length=4096)
+        a = run_mlm(spec, want_rows=True)
+        torch.cuda.synchronize()
+        o = oracle.radius_search(q, hq, r, hr, domain_min=mn, domain_max=mx, periodic=[True, False, True],
+                                 mode=mode, hash_map_length=4096)
+        np.testing.assert_array_equal(a.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(a.col.cpu().numpy(), o.col)

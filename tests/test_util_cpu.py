@@ -1,0 +1,24 @@
+"""CPU checks of the host-side helpers that mirror the reference's util
+module."""
+import numpy as np
+
+import oracle
+from torchcompactradius_amd.util import compute_h
+
+
+def test_compute_h_matches_oracle_on_random_domains():
+    rng = np.random.default_rng(0)
+    for _ in range(5000):
+        dim = int(rng.integers(1, 4))
+        mn = rng.uniform(-3, 1, dim).astype(np.float32)
+        mx = (mn + rng.uniform(0.1, 5, dim)).astype(np.float32)
+        h = np.float32(rng.uniform(0.001, 1.0))
+        a, b = np.float32(compute_h(mn, mx, h)), np.float32(oracle.compute_h(mn, mx, h))
+        assert a == b or (np.isnan(a) and np.isnan(b))
+
+
+def test_compute_h_workload_values():
+    # 2-D 1024-point workload: hCell 0.2 (10 x 10 cells); 1M-point 3-D: 0.046512
+    assert np.float32(compute_h([-1, -1], [1, 1], np.float32(np.sqrt(32 * (2 / 32) ** 2 / np.pi)))) == np.float32(0.2)
+    h = np.float32((3 * 50 * 0.02 ** 3 / (4 * np.pi)) ** (1 / 3))
+    assert abs(compute_h([-1] * 3, [1] * 3, h) - 0.046512) < 1e-6

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtcr_hip.so")
 
 MODES = {"symmetric": 0, "gather": 1, "scatter": 2, "superSymmetric": 3}
-TCR_OK, TCR_ERR_ARG, TCR_ERR_HIP, TCR_ERR_DOMAIN, TCR_ERR_SPACE = 0, -1, -2, -3, -4
+TCR_OK, TCR_ERR_ARG, TCR_ERR_HIP, TCR_ERR_DOMAIN, TCR_ERR_SPACE, TCR_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 
 
 class TcrCloud(ctypes.Structure):
@@ -47,6 +47,11 @@ class TcrError(RuntimeError):
         self.code = code
 
 
+class Unsupported(TcrError):
+    """Inputs the multi-level search does not take (the single-level search
+    gives the same output for them)."""
+
+
 _lib: Optional[ctypes.CDLL] = None
 
 # Every symbol include/tcr.h declares (checked by the ABI tests).
@@ -54,6 +59,7 @@ EXPORTED = (
     "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_launch",
     "tcr_search_finish", "tcr_search_export", "tcr_timing_enable", "tcr_timing_collect",
     "tcr_graph_capture", "tcr_graph_launch", "tcr_graph_destroy", "tcr_set_tuning", "tcr_get_tuning",
+    "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_last_error", "tcr_version",
 )
@@ -95,6 +101,13 @@ def load() -> ctypes.CDLL:
     L.tcr_set_tuning.argtypes = [ctypes.POINTER(TcrTuning)]
     L.tcr_get_tuning.restype = ctypes.c_int
     L.tcr_get_tuning.argtypes = [ctypes.POINTER(TcrTuning)]
+    L.tcr_mlm_workspace_size.restype = ctypes.c_size_t
+    L.tcr_mlm_workspace_size.argtypes = [cl]
+    L.tcr_mlm_count.restype = ctypes.c_int
+    L.tcr_mlm_count.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int32), vp]
+    L.tcr_mlm_fill.restype = ctypes.c_int
+    L.tcr_mlm_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp, vp]
     L.tcr_timing_enable.restype = ctypes.c_int
     L.tcr_timing_enable.argtypes = [i32]
     L.tcr_timing_collect.restype = ctypes.c_int
@@ -120,6 +133,8 @@ def check(rc: int) -> None:
         msg = load().tcr_last_error().decode(errors="replace")
         if rc == TCR_ERR_ARG:
             raise ValueError(msg)
+        if rc == TCR_ERR_UNSUPPORTED:
+            raise Unsupported(rc, msg)
         raise TcrError(rc, msg)
 
 

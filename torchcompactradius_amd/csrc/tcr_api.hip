@@ -1,5 +1,6 @@
 // C ABI of libtcr_hip.so (declared in include/tcr.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,7 +22,7 @@ static int64_t lintab_slots(int64_t nr) {
     return cap;
 }
 
-bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) {
+bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err, int lt_mult) {
     if (n < 1 || !clouds) {
         err = "need at least one cloud";
         return false;
@@ -47,7 +48,7 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err) 
         d.hb_off = b.hb_total;
         b.hb_total += h.hash_map_length;
         d.lt_off = b.lt_total;
-        d.lt_cap = lintab_slots(d.h.nr);
+        d.lt_cap = lintab_slots((int64_t)lt_mult / 2 * d.h.nr);
         b.lt_total += d.lt_cap;
         b.nq_total += h.nq;
         b.nr_total += d.h.nr;
@@ -154,9 +155,9 @@ struct Prepared {
 // Validates, carves the workspace and uploads descriptors and prefixes.
 static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup, const float *rpos,
                    const float *rsup, void *ws, size_t ws_bytes, hipStream_t s, Prepared &p, bool upload,
-                   bool check_data = true) {
+                   bool check_data = true, int lt_mult = 2) {
     std::string err;
-    if (!make_batch(clouds, n, p.b, err)) {
+    if (!make_batch(clouds, n, p.b, err, lt_mult)) {
         set_error(err);
         return TCR_ERR_ARG;
     }
@@ -472,6 +473,117 @@ int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds, void *workspace
                      false);
     if (rc) return rc;
     return finish_search(p, offsets, total_pairs, s);
+}
+
+size_t tcr_mlm_workspace_size(const tcr_cloud *cloud) {
+    Batch b;
+    std::string err;
+    if (!make_batch(cloud, 1, b, err, 4)) {
+        set_error(err);
+        return 0;
+    }
+    Work w;
+    return carve_work(b, nullptr, w);
+}
+
+int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos, const float *qsup,
+                  const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes, int64_t *offsets,
+                  int64_t *total_pairs, int32_t *levels_used, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!cloud || !offsets) {
+        set_error("cloud or offsets missing");
+        return TCR_ERR_ARG;
+    }
+    if (max_levels < 1 || max_levels > 4) {
+        set_error("max_levels must be 1..4");
+        return TCR_ERR_ARG;
+    }
+    if (cloud->mode == TCR_MODE_GATHER) {
+        set_error("multi-level search: gather mode bounds no radius by the reference supports");
+        return TCR_ERR_UNSUPPORTED;
+    }
+    if (cloud->search_range != 1) {
+        set_error("multi-level search: search_range must be 1");
+        return TCR_ERR_UNSUPPORTED;
+    }
+    Prepared p;
+    int rc = prepare(cloud, 1, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, true, true, 4);
+    if (rc) return rc;
+    rc = build_prep(p.b, p.w, s);
+    if (rc) return rc;
+    // the level count depends on the grid, which the device just computed
+    CloudDev cd;
+    TCR_HIP(hipMemcpyAsync(&cd, p.w.desc, sizeof(CloudDev), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    if (cd.err) {
+        set_error("a periodic dimension has no cells (support exceeds the domain)");
+        return TCR_ERR_DOMAIN;
+    }
+    const int dim = cd.h.dim;
+    uint64_t cells = 1;
+    for (int d = 0; d < dim; ++d) {
+        if (cd.nc[d] < 3) {
+            set_error("multi-level search: fewer than 3 cells in dimension " + std::to_string(d));
+            return TCR_ERR_UNSUPPORTED;
+        }
+        cells *= (uint64_t)cd.nc[d];
+    }
+    if (!(cd.hcell > 0.0f) || !std::isfinite(cd.hcell) || cells > (1ull << 29)) {
+        set_error("multi-level search: grid too large or degenerate");
+        return TCR_ERR_UNSUPPORTED;
+    }
+    int L = max_levels;
+    while (L > 1 && (cells << (dim * (L - 1))) > (1ull << 29)) --L;
+    rc = mlm_keys_launch(p.b, p.w, L, s);
+    if (rc) return rc;
+    rc = build_sorted_cells(p.b, p.w, s);
+    if (rc) return rc;
+    rc = mlm_groups_launch(p.b, p.w, s);
+    if (rc) return rc;
+    rc = mlm_query_launch(p.b, p.w, L, false, nullptr, nullptr, s);
+    if (rc) return rc;
+    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
+    if (rc) return rc;
+    Readback *rb = readback_slot();
+    if (!rb) {
+        set_error("hipHostMalloc of the readback slot failed");
+        return TCR_ERR_HIP;
+    }
+    TCR_HIP(hipMemcpyAsync(&rb->total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(&rb->err, p.w.misc + kMiscMlmFlags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    if (rb->err) {
+        set_error(std::string("multi-level search does not take these inputs:") +
+                  ((rb->err & 1) ? " non-finite or negative supports" : "") +
+                  ((rb->err & 2) ? " reference points outside the grid" : "") +
+                  ((rb->err & 4) ? " query points outside the grid" : ""));
+        return TCR_ERR_UNSUPPORTED;
+    }
+    if (total_pairs) *total_pairs = rb->total;
+    if (levels_used) *levels_used = L;
+    return TCR_OK;
+}
+
+int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels, const float *qpos, const float *qsup, const float *rpos,
+                 const float *rsup, void *workspace, size_t workspace_bytes, const int64_t *offsets, void *row,
+                 void *col, int32_t index_bytes, uint64_t *scratch, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    PairOut out;
+    if (!pair_out(row, col, index_bytes, out)) return TCR_ERR_ARG;
+    if (!cloud || !offsets || !col || !scratch) {
+        set_error("cloud, offsets, col or scratch missing");
+        return TCR_ERR_ARG;
+    }
+    if (levels < 1 || levels > 4) {
+        set_error("levels must be what tcr_mlm_count reported (1..4)");
+        return TCR_ERR_ARG;
+    }
+    Prepared p;
+    int rc = prepare(cloud, 1, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false, true, 4);
+    if (rc) return rc;
+    rc = mlm_query_launch(p.b, p.w, levels, true, offsets, scratch, s);
+    if (rc) return rc;
+    return mlm_sort_rows_launch(p.b, offsets, scratch, out, s);
 }
 
 int tcr_search_fill(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,

@@ -368,25 +368,24 @@ __global__ void k_cloud_cells(CloudDev *__restrict__ desc, int32_t B, const uint
 
 static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)((n + t - 1) / t); }
 
-int build_structure(const Batch &b, Work &w, hipStream_t s) {
-    const int32_t B = b.B;
-    int rc = TCR_OK;
+int build_prep(const Batch &b, Work &w, hipStream_t s) {
     // hMax + grid: one launch
     bool ref_side = false;
     for (const CloudDev &d : b.dev)
         if (!d.h.self_search && d.h.mode != TCR_MODE_GATHER && d.h.nr > 0) ref_side = true;
     const int nbq = (int)std::min<int64_t>(512, blocks_for(b.nq_total));
     const int nbr = ref_side ? (int)std::min<int64_t>(512, blocks_for(b.nr_total)) : 0;
-    hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, B,
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, b.B,
                        (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
                        w.misc);
     TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
+
+int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
     const int64_t n = b.nr_total;
     if (n == 0) return TCR_OK;
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w.desc, B, (const int64_t *)w.r_start, n,
-                       w.rwrap, w.keys0, w.vals0, w.pflags);
-    TCR_LAUNCH_CHECK();
-    rc = radix_sort(b, w, s);
+    int rc = radix_sort(b, w, s);
     if (rc) return rc;
     rc = fill_u32(w.lintab, 0xffffffffu, 2 * (size_t)b.lt_total, s);
     if (rc) return rc;
@@ -394,7 +393,7 @@ int build_structure(const Batch &b, Work &w, hipStream_t s) {
     if (rc) return rc;
     CellsArgs ca;
     ca.desc = w.desc;
-    ca.B = B;
+    ca.B = b.B;
     ca.ntiles = b.rt_total;
     ca.n = n;
     ca.t_start = w.rt_start;
@@ -409,9 +408,20 @@ int build_structure(const Batch &b, Work &w, hipStream_t s) {
     ca.lb = w.cell_lb;
     ca.misc = w.misc;
     hipLaunchKernelGGL(k_cells, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, ca);
-    hipLaunchKernelGGL(k_cloud_cells, dim3(1), dim3(64), 0, s, w.desc, B, (const uint32_t *)w.misc);
+    hipLaunchKernelGGL(k_cloud_cells, dim3(1), dim3(64), 0, s, w.desc, b.B, (const uint32_t *)w.misc);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
+}
+
+int build_structure(const Batch &b, Work &w, hipStream_t s) {
+    int rc = build_prep(b, w, s);
+    if (rc) return rc;
+    const int64_t n = b.nr_total;
+    if (n == 0) return TCR_OK;
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w.desc, b.B, (const int64_t *)w.r_start, n,
+                       w.rwrap, w.keys0, w.vals0, w.pflags);
+    TCR_LAUNCH_CHECK();
+    return build_sorted_cells(b, w, s);
 }
 
 // ------------------------------------------ reference-layout bucket tables --

@@ -30,7 +30,9 @@ enum : int {
     kMiscPrepTicket = 4, // last-block ticket of k_prep
     kMiscCellTicket = 5, // tile ticket of k_cells
     kMiscErr = 6,        // nonzero: some cloud has a degenerate grid
+    kMiscMlmFlags = 7,   // multi-level search: inputs it does not handle (see tcr_mlm.hip)
     kMiscSortTicket = 8, // tile tickets of the radix passes (kMaxPasses)
+    kMiscLevelMax = 12,  // multi-level search: ordered max support per level (4 words)
     kMiscWords = 16,
 };
 
@@ -147,7 +149,10 @@ struct RefOpArgs {
 int refop_launch(int dim, bool fill, const RefOpArgs &a, hipStream_t s);
 int hash_indices_launch(const int32_t *cells, int64_t n, int dim, uint32_t L, int32_t *out, hipStream_t s);
 
-bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err);
+// lt_mult: lin-table slots per reference point before rounding up to a power
+// of two (2 for the single-level search; the multi-level search also stores
+// its groups there and takes 4).
+bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err, int lt_mult = 2);
 size_t carve_work(const Batch &b, void *ws, Work &w);
 
 // Fills n 32-bit words with v (a kernel, so it is safe inside graph capture).
@@ -184,9 +189,22 @@ struct PairOut {
     }
 };
 
+int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s);
+
 // Structure build (hMax, grid, keys, sort, cells, lin table).  Expects the
 // header block (descriptors, prefixes, zeroed misc) uploaded.
 int build_structure(const Batch &b, Work &w, hipStream_t s);
+// Its stages: hMax + grid of every cloud (k_prep), and -- after the keys
+// (keys0/vals0/rwrap/pflags) are written -- sort, cells and lin table.
+int build_prep(const Batch &b, Work &w, hipStream_t s);
+int build_sorted_cells(const Batch &b, Work &w, hipStream_t s);
+
+// Multi-level search of one cloud (tcr_mlm.hip): level keys, group table,
+// count/fill traversal, per-row ordering.
+int mlm_keys_launch(const Batch &b, Work &w, int levels, hipStream_t s);
+int mlm_groups_launch(const Batch &b, Work &w, hipStream_t s);
+int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64_t *offsets, uint64_t *scratch,
+                     hipStream_t s);
 // Count pass (counts[] and the fill's masks); fill pass into offsets,
 // skipped on the device when offsets[nq_total] > capacity.
 int query_count(const Batch &b, Work &w, hipStream_t s);

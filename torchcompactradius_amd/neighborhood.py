@@ -13,7 +13,8 @@ from typing import List, Optional, Tuple, Union
 
 import torch
 
-from .search import CloudSpec, CompactHashMap, export_structure, periodic_mask, run_search, _host_floats
+from ._lib import Unsupported
+from .search import CloudSpec, CompactHashMap, export_structure, periodic_mask, run_mlm, run_search, _host_floats
 from .util import DomainDescription, PointCloud, SparseCOO, SparseCSR
 
 ALGORITHMS = ("naive", "small", "compact", "mlm")
@@ -51,8 +52,11 @@ def radiusSearch(
     indices, columns reference indices, row-sorted; within a row the order is
     the reference compact path's (cell stencil order, then original index).
 
-    Only ``algorithm='compact'`` is computed by this package; every search
-    runs on the GPU.  ``supportOverride`` maps a scalar support onto every
+    ``algorithm='compact'`` and ``'mlm'`` are computed by this package, every
+    search on the GPU.  ``'mlm'`` -- the multi-level search for variable
+    supports -- returns exactly the compact result (pairs and order); for
+    inputs it does not take (gather mode, points outside the grid, ...) and
+    with ``returnStructure`` it runs the compact search.  ``supportOverride`` maps a scalar support onto every
     point (the reference's fixed-support compact variant is not callable).
     ``indexDtype`` (keyword only, an extension) selects int64 row/col
     indices -- the reference's -- or int32 ones.
@@ -77,9 +81,9 @@ def radiusSearch(
         if referencePointCloud is not None and referencePointCloud.supports is not None:
             assert referencePointCloud.supports.shape[0] == referencePointCloud.positions.shape[0], \
                 "support.shape[0] != referencePositions.shape[0]"
-        if algorithm != "compact":
+        if algorithm not in ("compact", "mlm"):
             raise NotImplementedError(
-                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use algorithm='compact'")
+                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use 'compact' or 'mlm'")
 
         qpos = queryPointCloud.positions
         qsup = queryPointCloud.supports
@@ -99,7 +103,14 @@ def radiusSearch(
                   f"dim={dim} mode={mode} periodic_mask={pmask} hashMapLength={hashMapLength} format={format}")
         spec = CloudSpec(qpos=qpos, qsup=qsup, rpos=rpos, rsup=rsup, domain_min=mn, domain_max=mx,
                          periodic=pmask, mode=mode, hash_map_length=int(hashMapLength))
-        res = run_search([spec], want_rows=(format == "coo"), index_dtype=indexDtype)
+        res = None
+        if algorithm == "mlm" and not returnStructure:
+            try:
+                res = run_mlm(spec, want_rows=(format == "coo"), index_dtype=indexDtype)
+            except Unsupported:
+                res = None      # the single-level search gives the same output
+        if res is None:
+            res = run_search([spec], want_rows=(format == "coo"), index_dtype=indexDtype)
         nq = qpos.shape[0]
         nr = nq if rpos is None else rpos.shape[0]
         if format == "coo":

@@ -309,6 +309,36 @@ def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.in
     return SearchPlan(specs, want_rows, index_dtype).run()
 
 
+def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_levels: int = 4) -> "SearchResult":
+    """Multi-level search of one cloud (same output as ``run_search([spec])``).
+    Raises ``_lib.Unsupported`` for inputs it does not take."""
+    if index_dtype not in (torch.int64, torch.int32):
+        raise ValueError("index_dtype must be torch.int64 or torch.int32")
+    L = _lib.load()
+    device = spec.qpos.device
+    (qpos, qsup, rpos, rsup), q_offs, r_offs, nq, nr = _concat([spec], device)
+    clouds = _cloud_array([spec], q_offs, r_offs)
+    ws_bytes = L.tcr_mlm_workspace_size(clouds)
+    if ws_bytes == 0:
+        check(_lib.TCR_ERR_ARG)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    offsets = torch.empty(nq + 1, dtype=torch.int64, device=device)
+    total = ctypes.c_int64(0)
+    levels = ctypes.c_int32(0)
+    stream = _stream()
+    check(L.tcr_mlm_count(clouds, int(max_levels), ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
+                          ptr(offsets), ctypes.byref(total), ctypes.byref(levels), stream))
+    P = int(total.value)
+    col = torch.empty(P, dtype=index_dtype, device=device)
+    row = torch.empty(P, dtype=index_dtype, device=device) if want_rows else None
+    if P > 0:
+        scratch = col if index_dtype == torch.int64 else torch.empty(P, dtype=torch.int64, device=device)
+        check(L.tcr_mlm_fill(clouds, levels.value, ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
+                             ptr(offsets), ptr(row), ptr(col), 8 if index_dtype == torch.int64 else 4, ptr(scratch),
+                             stream))
+    return SearchResult(offsets, row, col, P, ws, clouds)
+
+
 def export_structure(res: SearchResult, cloud: int, spec: CloudSpec) -> CompactHashMap:
     """Copy one cloud's structure out of the workspace (returnStructure)."""
     L = _lib.load()

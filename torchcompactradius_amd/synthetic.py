@@ -96,3 +96,38 @@ def batched_clouds(seed: int = 4, n_clouds: int = 256, n_min: float = 1e3, n_max
                             np.full(dim, -1, np.float32), np.full(dim, 1, np.float32), periodic,
                             "gather", 1 << 16, "csr", meta={"dx": dx, "h": float(h), "neighbors": neighbors}))
     return out
+
+
+def adaptive_3d(n: int = 2_000_000, seed: int = 3, neighbors: int = 50, ratio: float = 8.0,
+                clusters: int = 8) -> Workload:
+    """Adaptive-support 3-D cloud: 70% of the points from `clusters` isotropic
+    Gaussians (sigma log-uniform in [0.02, 0.2], centres U[-0.7, 0.7]^3,
+    rejection-sampled into [-1, 1]^3) and 30% uniform in [-1, 1]^3; each
+    point's support is its distance to the `neighbors`-th nearest other point
+    (k-NN density estimate), clipped to [q, `ratio` q] with q the 1st
+    percentile of those distances.  Symmetric mode, non-periodic, COO,
+    self-search."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    n_g = int(0.7 * n)
+    centres = rng.uniform(-0.7, 0.7, (clusters, 3))
+    sig = np.exp(rng.uniform(math.log(0.02), math.log(0.2), clusters))
+    parts = []
+    for c in range(clusters):
+        m = n_g // clusters + (1 if c < n_g % clusters else 0)
+        got = np.empty((0, 3))
+        while got.shape[0] < m:
+            x = centres[c] + sig[c] * rng.standard_normal((2 * (m - got.shape[0]) + 16, 3))
+            x = x[np.all(np.abs(x) <= 1.0, axis=1)]
+            got = np.concatenate([got, x])
+        parts.append(got[:m])
+    parts.append(rng.uniform(-1.0, 1.0, (n - n_g, 3)))
+    pos = np.concatenate(parts).astype(np.float32)
+    d, _ = cKDTree(pos.astype(np.float64)).query(pos.astype(np.float64), k=neighbors + 1, workers=-1)
+    h = d[:, -1]
+    q = np.percentile(h, 1.0)
+    h = np.clip(h, q, ratio * q).astype(np.float32)
+    return Workload(f"adaptive-3d-{n}", pos, h, np.full(3, -1, np.float32), np.full(3, 1, np.float32),
+                    [False] * 3, "symmetric", 1 << 20, "coo",
+                    meta={"h_min": float(h.min()), "h_max": float(h.max()), "neighbors": neighbors})

@@ -85,3 +85,27 @@ def csr_to_coo(csr: SparseCSR) -> SparseCOO:
     counts = csr.rowEntries.to(torch.int64)
     row = torch.repeat_interleave(torch.arange(csr.numRows, device=csr.indices.device), counts)
     return SparseCOO(row, csr.indices, csr.numRows, csr.numCols)
+
+
+def compute_h(qMin, qMax, referenceSupport) -> float:
+    """Cell size of the compact grid for support hMax = referenceSupport
+    (util.py:28-79), in fp32: numCells = floor(extent / hMax) per dimension,
+    one cell fewer everywhere if any extent / (extent / numCells) overshoots,
+    a 1.0001 stretch if any extent / h is not integral, the largest h."""
+    mn = np.asarray(qMin, dtype=np.float32).reshape(-1)
+    mx = np.asarray(qMax, dtype=np.float32).reshape(-1)
+    hmax = np.float32(referenceSupport)
+    ext = (mx - mn).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nc = np.floor(ext / hmax).astype(np.float32)
+        h = (ext / nc).astype(np.float32)
+        if np.any((ext / h).astype(np.float32) - nc > np.float32(0)):
+            nc = (nc - np.float32(1)).astype(np.float32)
+            h = (ext / nc).astype(np.float32)
+        if np.any(np.ceil((ext / h).astype(np.float32)) > (ext / h).astype(np.float32)):
+            h = (h * np.float32(1.0001)).astype(np.float32)
+    out = h[0]
+    for v in h[1:]:
+        if v > out or np.isnan(v):
+            out = v
+    return float(np.float32(out))
