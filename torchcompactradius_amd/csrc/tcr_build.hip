@@ -78,14 +78,11 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
 // modes both.  Blocks [0, nbq) reduce slices of the concatenated query
 // supports, the rest slices of the reference supports; a block reduces per
 // cloud in registers and LDS and issues one atomicMax per cloud it touched.
-// The last block to finish computes every cloud's grid.
 __global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, int32_t B,
                                                  const int64_t *__restrict__ q_start, int64_t nq, int nbq,
-                                                 const int64_t *__restrict__ r_start, int64_t nr, int nbr,
-                                                 uint32_t *__restrict__ misc) {
+                                                 const int64_t *__restrict__ r_start, int64_t nr, int nbr) {
     __shared__ uint32_t s_max[kBlock / kWave];
     __shared__ int s_cloud[kBlock / kWave];
-    __shared__ bool s_last;
     const bool qside = (int)blockIdx.x < nbq;
     const int bi = qside ? (int)blockIdx.x : (int)blockIdx.x - nbq;
     const int nb = qside ? nbq : nbr;
@@ -99,7 +96,24 @@ __global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, in
         bool use = false;
         const float *sup = nullptr;
         int64_t c_lo = 0, c_hi = -1;
-        for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
+        // a slice inside one cloud (always, for one cloud) reads without a
+        // per-element cloud check, so the loads can be issued back to back
+        const int cs = g0 < g1 ? cloud_of(B, g0, start) : 0;
+        const bool one_cloud = g0 < g1 && start[cs + 1] >= g1;
+        if (one_cloud) {
+            const CloudDev &cd = desc[cs];
+            const int mode = cd.h.mode;
+            use = qside ? (mode != TCR_MODE_SCATTER || cd.h.self_search)
+                        : (mode != TCR_MODE_GATHER && !cd.h.self_search);
+            sup = qside ? cd.qs : cd.rs;
+            c = cs;
+            if (use && sup) {
+                const float *p = sup - start[cs];
+#pragma unroll 4
+                for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) m = max(m, float_to_ordered(p[g]));
+            }
+        }
+        for (int64_t g = g0 + threadIdx.x; !one_cloud && g < g1; g += kBlock) {
             if (g >= c_hi) {
                 if (c >= 0 && m) atomicMax(&desc[c].hmax_ord, m);
                 c = cloud_of(B, g, start);
@@ -144,19 +158,16 @@ __global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, in
             }
         }
     }
-    // last block: grids of every cloud.  One release fence per block after
-    // the barrier orders the block's atomicMax updates before its ticket
-    // (a fence per thread would write back the L2 once per wavefront).
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(&misc[kMiscPrepTicket], 1u) == gridDim.x - 1;
-        if (s_last) __threadfence();
-    }
-    __syncthreads();
-    if (!s_last) return;
-    for (int c = threadIdx.x; c < B; c += kBlock) {
-        grid_of(desc[c], __hip_atomic_load(&desc[c].hmax_ord, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Grids of every cloud from the reduced hMax (a kernel of its own: the
+// launch boundary orders it after every block's atomicMax, where a
+// last-block ticket would need an agent-scope release fence per block --
+// an L2 write-back on this GPU).
+__global__ void __launch_bounds__(kBlock) k_grid(CloudDev *__restrict__ desc, int32_t B,
+                                                 uint32_t *__restrict__ misc) {
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < B; c += gridDim.x * kBlock) {
+        grid_of(desc[c], desc[c].hmax_ord);
         if (desc[c].err) atomicOr(&misc[kMiscErr], 1u);
     }
 }
@@ -255,9 +266,6 @@ struct CellsArgs {
     uint32_t *misc;
 };
 
-constexpr uint64_t kCellAgg = 1ull << 32;
-constexpr uint64_t kCellInc = 2ull << 32;
-
 __global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
     constexpr int kItems = kSortTile / kSortThreads;  // 16 consecutive slots per thread
     __shared__ uint32_t s_wsum[kSortThreads / kWave];
@@ -303,22 +311,11 @@ __global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
         uint64_t *lb = a.lb + tile;
         uint64_t excl = 0;
         if (tile == 0) {
-            __hip_atomic_store(lb, kCellInc | ttot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lb_store(lb, kLbInc | ttot);
         } else {
-            __hip_atomic_store(lb, kCellAgg | ttot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t t = tile - 1;
-            while (true) {
-                const uint64_t v = __hip_atomic_load(a.lb + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t flag = v & ~0xffffffffull;
-                if (flag == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)v;
-                if (flag == kCellInc) break;
-                --t;
-            }
-            __hip_atomic_store(lb, kCellInc | (excl + ttot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lb_store(lb, kLbAgg | ttot);
+            excl = look_back(a.lb, 1, tile - 1, 0);
+            lb_store(lb, kLbInc | (excl + ttot));
         }
         s_excl = excl;
         if (tile == first_tile) a.desc[c].cell_off = (int64_t)excl;
@@ -332,6 +329,7 @@ __global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
     const CloudDev &cd = a.desc[c];
     uint64_t *tab = a.lintab + cd.lt_off;
     const uint64_t cap = (uint64_t)cd.lt_cap;
+    const bool dense = cell_index_dense(cd);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (!((fl >> k) & 1u)) continue;
@@ -343,10 +341,14 @@ __global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
         const float4 p = a.sorted[s];
         a.cell_hash[cid] = cd.h.dim == 1 ? cell_bucket<1>(cd, p) : cd.h.dim == 2 ? cell_bucket<2>(cd, p)
                                                                                  : cell_bucket<3>(cd, p);
-        const uint64_t e = ((uint64_t)cid << 32) | (uint32_t)lin;
-        uint64_t h = lintab_hash(lin) & (cap - 1);
-        while (atomicCAS((unsigned long long *)&tab[h], ~0ull, (unsigned long long)e) != ~0ull)
-            h = (h + 1) & (cap - 1);
+        if (dense) {
+            ((int32_t *)tab)[lin] = (int32_t)cid;   // lin in [0, cells) for a regular cloud
+        } else {
+            const uint64_t e = ((uint64_t)cid << 32) | (uint32_t)lin;
+            uint64_t h = lintab_hash(lin) & (cap - 1);
+            while (atomicCAS((unsigned long long *)&tab[h], ~0ull, (unsigned long long)e) != ~0ull)
+                h = (h + 1) & (cap - 1);
+        }
         ++cid;
     }
 }
@@ -368,6 +370,19 @@ __global__ void k_cloud_cells(CloudDev *__restrict__ desc, int32_t B, const uint
 
 static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)((n + t - 1) / t); }
 
+// Empties every cloud's cell index: the whole open-addressing table, or the
+// first grid-cells int32 entries of a dense one.
+__global__ void __launch_bounds__(kBlock) k_clear_index(const CloudDev *__restrict__ desc, int32_t B,
+                                                        const int64_t *__restrict__ lt_start, int64_t total,
+                                                        uint64_t *__restrict__ lintab) {
+    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += (int64_t)gridDim.x * kBlock) {
+        const int c = cloud_of(B, g, lt_start);
+        const CloudDev &cd = desc[c];
+        const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
+        if (g - lt_start[c] < need) lintab[g] = ~0ull;
+    }
+}
+
 int build_prep(const Batch &b, Work &w, hipStream_t s) {
     // hMax + grid: one launch
     bool ref_side = false;
@@ -376,8 +391,9 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
     const int nbq = (int)std::min<int64_t>(512, blocks_for(b.nq_total));
     const int nbr = ref_side ? (int)std::min<int64_t>(512, blocks_for(b.nr_total)) : 0;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, b.B,
-                       (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
-                       w.misc);
+                       (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr);
+    hipLaunchKernelGGL(k_grid, dim3((unsigned)std::min<int64_t>(64, (b.B + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       w.desc, b.B, w.misc);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
@@ -387,8 +403,12 @@ int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
     if (n == 0) return TCR_OK;
     int rc = radix_sort(b, w, s);
     if (rc) return rc;
-    rc = fill_u32(w.lintab, 0xffffffffu, 2 * (size_t)b.lt_total, s);
-    if (rc) return rc;
+    if (b.lt_total > 0) {
+        const int64_t blocks = std::min<int64_t>(8192, (b.lt_total + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_clear_index, dim3((unsigned)blocks), dim3(kBlock), 0, s, (const CloudDev *)w.desc, b.B,
+                           (const int64_t *)w.lt_start, b.lt_total, w.lintab);
+        TCR_LAUNCH_CHECK();
+    }
     rc = fill_u32(w.cell_lb, 0u, 2 * (size_t)b.rt_total, s);
     if (rc) return rc;
     CellsArgs ca;

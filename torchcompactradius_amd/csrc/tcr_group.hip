@@ -1,12 +1,12 @@
 // Wavefront-per-cell count and fill passes for self-search clouds (the query
 // set is the reference set), searchRange 1.
 //
-// 1. k_stencil (one thread per cell and stencil offset) resolves the 3^d
-//    neighbour cells of every occupied cell through the compact hash table,
-//    with the lookup semantics of queryHashMap (cpp/neighborhood.h:35-58):
-//    periodic coordinates through pymod, non-periodic ones unclamped, the
-//    bucket scan matching only the linear index.  It also packs a 32-byte
-//    record per cell (points, cloud, state, centre).
+// 1. k_stencil (one thread per occupied cell) resolves the 3^d neighbour
+//    cells of every occupied cell through the cell index, with the lookup
+//    semantics of queryHashMap (cpp/neighborhood.h:35-58): periodic
+//    coordinates wrapped, non-periodic ones unclamped, a class matching only
+//    its linear index and bucket.  It also packs a 32-byte record per cell
+//    (points, cloud, state, centre).
 // 2. k_count_small (one wavefront per cell with <= kSmallChunk candidates):
 //    the lanes lay the candidates out in the reference's order -- stencil
 //    offset order (dim 0 slowest), then sorted position -- over rounds of 64
@@ -35,6 +35,8 @@
 // sorted copy, or whose cell lies outside [0, numCells), are "strays" and go
 // to the per-query kernel, as do all points of a cell whose first point is
 // outside the grid.
+#include <algorithm>
+
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -159,57 +161,98 @@ __device__ __forceinline__ bool cell_coords(const CloudDev &cd, float4 p0, int32
     return inrange;
 }
 
-// Stencil cell o of the cell at q: (first global sorted slot, count), or
-// (0, 0) when the bucket lookup of the reference finds no such cell.
+// All 3^DIM stencil cells of one cell: the periodic neighbours of an in-grid
+// coordinate are q - 1 + n or q + 1 - n (a fast cloud has >= 6 cells in every
+// periodic dimension).  In a cloud whose points all lie in the grid, a
+// neighbour whose coordinates all lie in the grid is its class's own cell,
+// whose bucket is hash(c); only coordinates outside the grid (non-periodic
+// borders, whose linear index may alias another cell), and every lookup of a
+// cloud with points outside the grid (whose classes may start with such a
+// point), need the bucket check of the reference lookup.
 template <int DIM>
-__device__ __forceinline__ int2 stencil_one(const GroupArgs &a, const CloudDev &cd, const int32_t *q, int o) {
-    int32_t cc[3];
-    int32_t rem = o;
+__device__ __forceinline__ void stencil_cell(const GroupArgs &a, const CloudDev &cd, const int32_t *q,
+                                             int2 *__restrict__ out) {
+    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    int32_t cc[3][3];
+    bool in[3][3];
 #pragma unroll
-    for (int d = DIM - 1; d >= 0; --d) {
-        const int32_t off = (rem % 3) - 1;
-        rem /= 3;
-        cc[d] = ((cd.h.periodic_mask >> d) & 1u) ? pymod(q[d] + off, cd.nc[d]) : q[d] + off;
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cc[d][k] = 0;
+            in[d][k] = true;
+        }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int32_t n = cd.nc[d];
+        const bool per = (cd.h.periodic_mask >> d) & 1u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t u = q[d] + k - 1;
+            if (per) u = u < 0 ? u + n : (u >= n ? u - n : u);
+            cc[d][k] = u;
+            in[d][k] = u >= 0 && u < n;
+        }
     }
-    const int32_t lin = linear_index<DIM>(cc, cd.nc);
-    const uint32_t hb = hash_index<DIM>(cc, cd.h.hash_map_length);
-    const int32_t cid = lintab_find(a.lintab + cd.lt_off, (uint64_t)cd.lt_cap, lin);
-    if (cid < 0 || a.cell_hash[cid] != hb) return make_int2(0, 0);
-    const uint32_t b = a.cell_begin[cid];
-    return make_int2((int32_t)b, (int32_t)(a.cell_begin[cid + 1] - b));
+    const CellIndex idx = cell_index(cd, a.lintab);
+    const uint32_t L = cd.h.hash_map_length;
+#pragma unroll
+    for (int o = 0; o < kOff; ++o) {
+        // offset o: dim 0 slowest, as the reference's offset list
+        int32_t c[3] = {0, 0, 0};
+        bool all_in = true;
+        int rem = o;
+#pragma unroll
+        for (int d = DIM - 1; d >= 0; --d) {
+            const int k = rem % 3;
+            rem /= 3;
+            c[d] = cc[d][k];
+            all_in = all_in && in[d][k];
+        }
+        const int32_t cid = idx.find(linear_index<DIM>(c, cd.nc));
+        bool ok = cid >= 0;
+        if (ok && (!all_in || cd.irregular)) ok = a.cell_hash[cid] == hash_index<DIM>(c, L);
+        int2 r = make_int2(0, 0);
+        if (ok) {
+            const uint32_t b = a.cell_begin[cid];
+            r = make_int2((int32_t)b, (int32_t)(a.cell_begin[cid + 1] - b));
+        }
+        out[o] = r;
+    }
+#pragma unroll
+    for (int o = kOff; o < kStencil; ++o) out[o] = make_int2(0, 0);
 }
 
+// One thread per occupied cell: its record and its resolved stencil.
 __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
     const uint32_t ncells = a.misc[kMiscCells];
-    const int64_t n = (int64_t)ncells * kStencil;
-    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < n; g += (int64_t)gridDim.x * 256) {
-        const uint32_t cid = (uint32_t)(g / kStencil);
-        const int o = (int)(g - (int64_t)cid * kStencil);
+    for (int64_t cid = (int64_t)blockIdx.x * 256 + threadIdx.x; cid < ncells; cid += (int64_t)gridDim.x * 256) {
         const uint32_t c = a.cell_cloud[cid];
         const CloudDev &cd = a.desc[c];
         const int dim = cd.h.dim;
-        const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
-        const uint32_t gb = a.cell_begin[cid];
+        const uint32_t gb = a.cell_begin[cid], ge = a.cell_begin[cid + 1];
+        int2 *out = a.ranges + cid * kStencil;
         if (!group_owned(cd)) {
-            if (o == 0) a.recs[cid] = CellRec{gb, a.cell_begin[cid + 1], c, kCellSkip, make_float4(0, 0, 0, 0)};
+            a.recs[cid] = CellRec{gb, ge, c, kCellSkip, make_float4(0, 0, 0, 0)};
             continue;
         }
         const float4 p0 = a.sorted[gb];
-        int32_t q[3];
+        int32_t q[3] = {0, 0, 0};
         const bool inrange = dim == 1 ? cell_coords<1>(cd, p0, q) : dim == 2 ? cell_coords<2>(cd, p0, q)
                                                                              : cell_coords<3>(cd, p0, q);
         const bool fast = inrange && cd.fast;
-        if (o == 0) {
-            float cen[3] = {0.f, 0.f, 0.f};
-            for (int d = 0; d < dim; ++d) cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
-            a.recs[cid] = CellRec{gb, a.cell_begin[cid + 1], c, fast ? kCellFast : kCellStray,
-                                  make_float4(cen[0], cen[1], cen[2], 0.0f)};
+        float cen[3] = {0.f, 0.f, 0.f};
+        for (int d = 0; d < dim; ++d) cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
+        a.recs[cid] = CellRec{gb, ge, c, fast ? kCellFast : kCellStray, make_float4(cen[0], cen[1], cen[2], 0.0f)};
+        if (!fast) {
+            for (int o = 0; o < kStencil; ++o) out[o] = make_int2(0, 0);
+            continue;
         }
-        int2 r = make_int2(0, 0);
-        if (fast && o < noff)
-            r = dim == 1 ? stencil_one<1>(a, cd, q, o) : dim == 2 ? stencil_one<2>(a, cd, q, o)
-                                                                  : stencil_one<3>(a, cd, q, o);
-        a.ranges[g] = r;
+        switch (dim) {
+            case 1: stencil_cell<1>(a, cd, q, out); break;
+            case 2: stencil_cell<2>(a, cd, q, out); break;
+            default: stencil_cell<3>(a, cd, q, out); break;
+        }
     }
 }
 
@@ -816,8 +859,11 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
         if (!fill) {
             // misc[kMiscStrays] and misc[kMiscBig] start at zero: the
             // header upload clears all of misc
+            // one thread per cell: enough workgroups for every occupied cell
+            // (at most one cell per reference point)
             const int32_t sw = tuning().stencil_workgroups;
-            hipLaunchKernelGGL(k_stencil, dim3(sw > 0 ? (unsigned)sw : 2048u), dim3(256), 0, s, ga);
+            const unsigned sgrid = sw > 0 ? (unsigned)sw : (unsigned)std::max<int64_t>(1, (b.nr_total + 255) / 256);
+            hipLaunchKernelGGL(k_stencil, dim3(sgrid), dim3(256), 0, s, ga);
         } else {
             hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((b.nr_total + 255) / 256)), dim3(256), 0, s,
                                (const CloudDev *)w.desc, b.B, (const int64_t *)w.r_start,

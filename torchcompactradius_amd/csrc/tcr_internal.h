@@ -10,8 +10,8 @@
 
 namespace tcr {
 
-constexpr int kSortThreads = 256;          // 4 wavefronts per sort tile
-constexpr int kSortItems = 16;             // keys per thread
+constexpr int kSortThreads = 1024;         // 16 wavefronts per sort tile
+constexpr int kSortItems = 4;              // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
@@ -27,7 +27,6 @@ enum : int {
     kMiscStrays = 1,     // stray queries listed by the count pass
     kMiscBig = 2,        // cells with more candidates than one register chunk
     kMiscPasses = 3,     // radix passes of this search
-    kMiscPrepTicket = 4, // last-block ticket of k_prep
     kMiscCellTicket = 5, // tile ticket of k_cells
     kMiscErr = 6,        // nonzero: some cloud has a degenerate grid
     kMiscMlmFlags = 7,   // multi-level search: inputs it does not handle (see tcr_mlm.hip)
@@ -215,6 +214,47 @@ int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // positions gathered by its last pass.
 int radix_sort(const Batch &b, Work &w, hipStream_t s);
 
+// Decoupled look-back state: a 64-bit word per (tile[, digit]) holding a
+// 32-bit count and a flag -- 0 not published, aggregate of the tile alone, or
+// inclusive prefix up to and including the tile.
+constexpr uint64_t kLbAgg = 1ull << 32;
+constexpr uint64_t kLbInc = 2ull << 32;
+__device__ __forceinline__ uint64_t lb_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Exclusive prefix over tiles [first, t]: sums the published aggregates back
+// to the nearest inclusive value, reading kLookBack predecessors per step
+// (wider windows measured slower on the 1M-point sort); tiles that have not
+// published yet are waited for.  Tile `first` always publishes an inclusive
+// value, so the walk ends there at the latest.
+constexpr int kLookBack = 1;
+__device__ __forceinline__ uint64_t look_back(const uint64_t *lb, int64_t stride, int64_t t, int64_t first) {
+    uint64_t excl = 0;
+    while (true) {
+        const int cnt = (int)min((int64_t)kLookBack, t - first + 1);
+        uint64_t v[kLookBack];
+#pragma unroll
+        for (int k = 0; k < kLookBack; ++k) v[k] = k < cnt ? lb_load(lb + (t - k) * stride) : 0ull;
+        int k = 0;
+        bool done = false;
+        for (; k < cnt; ++k) {
+            const uint64_t flag = v[k] & ~0xffffffffull;
+            if (flag == 0) break;  // not published yet: resume from this tile
+            excl += (uint32_t)v[k];
+            if (flag == kLbInc) {
+                done = true;
+                break;
+            }
+        }
+        if (done) return excl;
+        t -= k;
+        if (k < cnt) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // lin -> cell lookup with the reference's bucket semantics: the class with
 // linear index `lin` answers a lookup for cell coordinates whose bucket is
 // `bucket` only if the class lives in that bucket (its first point's cell
@@ -228,6 +268,39 @@ __device__ __forceinline__ int32_t lintab_find(const uint64_t *__restrict__ tab,
         if ((int32_t)(uint32_t)e == lin) return (int32_t)(e >> 32);
         h = (h + 1) & (cap - 1);
     }
+}
+
+__device__ __forceinline__ uint64_t grid_cells(const CloudDev &cd) {
+    uint64_t P = 1;
+    for (int d = 0; d < cd.h.dim; ++d) P *= (uint64_t)(cd.nc[d] > 0 ? cd.nc[d] : 1);
+    return P;
+}
+
+// A cloud's cell index: when every reference point lies in the grid and the
+// grid has no more cells than the lin-table memory holds int32 entries, a
+// dense int32 array indexed by the linear index (one load, L2-resident for
+// dense clouds); the open-addressing table otherwise.  Both answer "the cell
+// with linear index lin", so lookups keep the bucket check on top.
+struct CellIndex {
+    const uint64_t *tab;
+    uint64_t cap, P;
+    bool dense;
+    __device__ __forceinline__ int32_t find(int32_t lin) const {
+        if (dense) return (uint32_t)lin < P ? ((const int32_t *)tab)[lin] : -1;
+        return lintab_find(tab, cap, lin);
+    }
+};
+__device__ __forceinline__ bool cell_index_dense(const CloudDev &cd) {
+    const uint64_t P = grid_cells(cd);
+    return !cd.irregular && P <= 2 * (uint64_t)cd.lt_cap && P < (1ull << 31);
+}
+__device__ __forceinline__ CellIndex cell_index(const CloudDev &cd, const uint64_t *lintab) {
+    CellIndex ci;
+    ci.tab = lintab + cd.lt_off;
+    ci.cap = (uint64_t)cd.lt_cap;
+    ci.P = grid_cells(cd);
+    ci.dense = cell_index_dense(cd);
+    return ci;
 }
 
 }  // namespace tcr

@@ -56,12 +56,11 @@ struct BucketLookup {
 // exists and lives in bucket hb -- the same answer as the bucket scan.
 // Returns global sorted slots minus the cloud's first slot.
 struct LinLookup {
-    const uint64_t *tab;
-    uint64_t cap;
+    CellIndex index;
     const uint32_t *cell_hash, *cell_begin;
     int64_t rs;
     __device__ __forceinline__ bool find(int32_t lin, uint32_t hb, int64_t &b, int64_t &e) const {
-        const int32_t cid = lintab_find(tab, cap, lin);
+        const int32_t cid = index.find(lin);
         if (cid < 0 || cell_hash[cid] != hb) return false;
         b = (int64_t)cell_begin[cid] - rs;
         e = (int64_t)cell_begin[cid + 1] - rs;
@@ -168,7 +167,7 @@ __device__ __forceinline__ void query_one(const GenericArgs &a, const CloudDev &
     load_query<DIM>(cd, local, xi, hi);
     const Geom gm = cloud_geom(cd);
     const int64_t rs = a.r_start[c];
-    const LinLookup lk{a.lintab + cd.lt_off, (uint64_t)cd.lt_cap, a.cell_hash, a.cell_begin, rs};
+    const LinLookup lk{cell_index(cd, a.lintab), a.cell_hash, a.cell_begin, rs};
     const Float4Cands cands{a.sorted + rs};
     if (!FILL) {
         a.counts[g] = (int32_t)traverse<DIM>(gm, lk, cands, xi, hi, [](int64_t, int64_t) {});

@@ -5,7 +5,7 @@
 //
 // Onesweep layout: one histogram kernel counts the digits of every pass for
 // every cloud, then one kernel per pass ranks a tile of 4096 keys in LDS
-// (wave64 multi-split with 8 ballots per round), finds the tile's global
+// (16 wavefronts, wave64 multi-split with 8 ballots per round), finds the tile's global
 // offset per digit by decoupled look-back over the earlier tiles of its
 // cloud, and scatters.  Tiles are numbered by an atomic ticket, so a tile
 // only ever waits for tiles that are already running.  The number of passes
@@ -46,12 +46,6 @@ __device__ __forceinline__ TileInfo tile_info(int64_t t, int32_t B, const int64_
 // passes); any other ("wide") cloud flips the sign bit and takes all passes.
 __device__ __forceinline__ uint32_t eff_key(uint32_t lin, bool wide) { return wide ? (lin ^ 0x80000000u) : lin; }
 
-__device__ __forceinline__ uint64_t grid_cells(const CloudDev &cd) {
-    uint64_t P = 1;
-    for (int d = 0; d < cd.h.dim; ++d) P *= (uint64_t)(cd.nc[d] > 0 ? cd.nc[d] : 1);
-    return P;
-}
-
 // Points outside the grid (INT32_MIN coordinates) or more than 2^31 cells
 // (linear indices wrap to negative values).
 __device__ __forceinline__ bool cloud_wide(const CloudDev &cd) {
@@ -89,16 +83,6 @@ k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restr
         if (v) atomicAdd(&hc[k], v);
     }
     if (threadIdx.x == 0) atomicMax(&misc[kMiscPasses], cloud_passes(cd));
-}
-
-constexpr uint64_t kFlagAgg = 1ull << 32;
-constexpr uint64_t kFlagInc = 2ull << 32;
-
-__device__ __forceinline__ uint64_t lb_load(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct SortPassArgs {
@@ -166,9 +150,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
         __asm__ volatile("" ::: "memory");
     }
     __syncthreads();
-    // thread d: tile count of digit d, wave prefixes, look-back, digit base
-    {
-        const int d = threadIdx.x;  // kSortThreads == kRadix
+    // thread d < kRadix: tile count of digit d, wave prefixes, look-back,
+    // digit base
+    if (threadIdx.x < kRadix) {
+        const int d = threadIdx.x;
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
@@ -178,24 +163,12 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
         }
         uint64_t *lb = a.lb + tile * kRadix + d;
         if (tile == ti.first) {
-            lb_store(lb, kFlagInc | run);
+            lb_store(lb, kLbInc | run);
             tile_excl[d] = 0;
         } else {
-            lb_store(lb, kFlagAgg | run);
-            uint64_t excl = 0;
-            int64_t t = tile - 1;
-            while (true) {
-                const uint64_t v = lb_load(a.lb + t * kRadix + d);
-                const uint64_t flag = v & ~0xffffffffull;
-                if (flag == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)v;
-                if (flag == kFlagInc) break;
-                --t;
-            }
-            lb_store(lb, kFlagInc | (excl + run));
+            lb_store(lb, kLbAgg | run);
+            const uint64_t excl = look_back(a.lb + d, kRadix, tile - 1, ti.first);
+            lb_store(lb, kLbInc | (excl + run));
             tile_excl[d] = (uint32_t)excl;
         }
         dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + d];
