@@ -373,13 +373,14 @@ static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)
 // Empties every cloud's cell index: the whole open-addressing table, or the
 // first grid-cells int32 entries of a dense one.
 __global__ void __launch_bounds__(kBlock) k_clear_index(const CloudDev *__restrict__ desc, int32_t B,
-                                                        const int64_t *__restrict__ lt_start, int64_t total,
+                                                        const int64_t *__restrict__ lt_start,
                                                         uint64_t *__restrict__ lintab) {
-    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += (int64_t)gridDim.x * kBlock) {
-        const int c = cloud_of(B, g, lt_start);
+    for (int c = 0; c < B; ++c) {
         const CloudDev &cd = desc[c];
         const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
-        if (g - lt_start[c] < need) lintab[g] = ~0ull;
+        uint64_t *tab = lintab + lt_start[c];
+        for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < need; g += (int64_t)gridDim.x * kBlock)
+            tab[g] = ~0ull;
     }
 }
 
@@ -404,9 +405,9 @@ int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
     int rc = radix_sort(b, w, s);
     if (rc) return rc;
     if (b.lt_total > 0) {
-        const int64_t blocks = std::min<int64_t>(8192, (b.lt_total + kBlock - 1) / kBlock);
+        const int64_t blocks = std::min<int64_t>(2048, (b.lt_total + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_clear_index, dim3((unsigned)blocks), dim3(kBlock), 0, s, (const CloudDev *)w.desc, b.B,
-                           (const int64_t *)w.lt_start, b.lt_total, w.lintab);
+                           (const int64_t *)w.lt_start, w.lintab);
         TCR_LAUNCH_CHECK();
     }
     rc = fill_u32(w.cell_lb, 0u, 2 * (size_t)b.rt_total, s);

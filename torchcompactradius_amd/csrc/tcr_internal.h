@@ -13,9 +13,9 @@ namespace tcr {
 constexpr int kSortThreads = 1024;         // 16 wavefronts per sort tile
 constexpr int kSortItems = 4;              // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
-constexpr int kRadixBits = 8;
+constexpr int kRadixBits = 9;              // 2 passes for grids of up to 2^18 cells
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kMaxPasses = 4;              // 32-bit keys
+constexpr int kMaxPasses = 4;              // 32-bit keys (36 bits of digits)
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;

@@ -5,7 +5,8 @@
 //
 // Onesweep layout: one histogram kernel counts the digits of every pass for
 // every cloud, then one kernel per pass ranks a tile of 4096 keys in LDS
-// (16 wavefronts, wave64 multi-split with 8 ballots per round), finds the tile's global
+// (16 wavefronts, 9-bit digits, wave64 multi-split with 9 ballots per
+// round), finds the tile's global
 // offset per digit by decoupled look-back over the earlier tiles of its
 // cloud, and scatters.  Tiles are numbered by an atomic ticket, so a tile
 // only ever waits for tiles that are already running.  The number of passes
@@ -107,10 +108,12 @@ struct SortPassArgs {
 
 __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     constexpr int kWaves = kSortThreads / kWave;
-    constexpr int kPerWave = kSortTile / kWaves;  // 1024
+    constexpr int kPerWave = kSortTile / kWaves;  // 256
+    static_assert(kRadix <= kSortThreads, "one thread per digit");
     __shared__ uint32_t wcnt[kWaves][kRadix];
     __shared__ uint32_t dbase[kRadix];      // digit base of the cloud in this pass
     __shared__ uint32_t tile_excl[kRadix];  // keys of this digit in earlier tiles of the cloud
+    __shared__ uint32_t s_wtot[kRadix / kWave];
     __shared__ int64_t s_tile;
     const uint32_t npass = a.misc[kMiscPasses];
     if ((uint32_t)a.pass >= npass) return;
@@ -174,13 +177,26 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
         dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + d];
     }
     __syncthreads();
-    // exclusive scan of the cloud's digit counts (one thread)
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int d = 0; d < kRadix; ++d) {
-            const uint32_t c = dbase[d];
-            dbase[d] = run;
-            run += c;
+    // exclusive scan of the cloud's digit counts: wavefront scans of 64
+    // digits, then the wavefront totals
+    {
+        const int d = threadIdx.x;
+        uint32_t incl = 0, c = 0;
+        if (d < kRadix) {
+            c = dbase[d];
+            incl = c;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o, kWave);
+                if (lane >= o) incl += v;
+            }
+            if (lane == kWave - 1) s_wtot[wave] = incl;
+        }
+        __syncthreads();
+        if (d < kRadix) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; ++w) pre += s_wtot[w];
+            dbase[d] = pre + incl - c;
         }
     }
     __syncthreads();
