@@ -211,6 +211,12 @@ int tcr_build_neighbor_list(const int32_t *counts, const int32_t *offsets, int64
                             const uint8_t *periodicity, int32_t dim, int32_t mode,
                             int64_t *list_i, int64_t *list_j, void *stream);
 
+/* The reference's CSR arrays from int64 offsets[nq + 1] (stream-ordered, so
+ * callable right after tcr_search_launch): indptr int32[nq + 1] and
+ * rowEntries int64[nq] (either may be NULL) -- the SparseCSR of
+ * py/neighborhood.py:303-305 / py/util.py:335-358. */
+int tcr_csr_arrays(const int64_t *offsets, int64_t nq, int32_t *indptr, int64_t *row_entries, void *stream);
+
 /* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
  * (no counterpart in the reference; a benchmark hook).  While enabled, each
  * launch records HIP events on its stream around the count phase (build,

@@ -59,7 +59,7 @@ EXPORTED = (
     "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_launch",
     "tcr_search_finish", "tcr_search_export", "tcr_timing_enable", "tcr_timing_collect",
     "tcr_graph_capture", "tcr_graph_launch", "tcr_graph_destroy", "tcr_set_tuning", "tcr_get_tuning",
-    "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill",
+    "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill", "tcr_csr_arrays",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_last_error", "tcr_version",
 )
@@ -108,6 +108,8 @@ def load() -> ctypes.CDLL:
                                 ctypes.POINTER(ctypes.c_int32), vp]
     L.tcr_mlm_fill.restype = ctypes.c_int
     L.tcr_mlm_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp, vp]
+    L.tcr_csr_arrays.restype = ctypes.c_int
+    L.tcr_csr_arrays.argtypes = [vp, i64, vp, vp, vp]
     L.tcr_timing_enable.restype = ctypes.c_int
     L.tcr_timing_enable.argtypes = [i32]
     L.tcr_timing_collect.restype = ctypes.c_int

@@ -201,15 +201,26 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
     return TCR_OK;
 }
 
-// Per-thread pinned readback slots (total pair count, error word).
+// Per-thread pinned readback slots (total pair count, error word), host
+// coherent so that a kernel's stores reach the host without a copy.
 struct Readback {
     int64_t total;
     uint32_t err;
 };
 static Readback *readback_slot() {
     static thread_local Readback *slot = nullptr;
-    if (!slot && hipHostMalloc((void **)&slot, sizeof(Readback), hipHostMallocDefault) != hipSuccess) slot = nullptr;
+    if (!slot && hipHostMalloc((void **)&slot, sizeof(Readback), hipHostMallocCoherent) != hipSuccess) slot = nullptr;
     return slot;
+}
+
+// One launch in place of two device-to-host copies.
+__global__ void k_readback(const int64_t *__restrict__ total, const uint32_t *__restrict__ err,
+                           Readback *__restrict__ rb) {
+    if (threadIdx.x == 0) {
+        rb->total = *total;
+        rb->err = *err;
+        __threadfence_system();
+    }
 }
 
 // Optional per-thread phase timing: HIP events recorded on the search
@@ -221,11 +232,18 @@ struct PhaseEvents {
 struct Timing {
     bool on = false;
     std::vector<PhaseEvents> pending;
+    std::vector<hipEvent_t> pool;   // collected events, reused
 };
 static thread_local Timing g_timing;
 
 static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
-    if (hipEventCreate(&pe.e[k]) != hipSuccess) return nullptr;
+    if (!g_timing.pool.empty()) {
+        pe.e[k] = g_timing.pool.back();
+        g_timing.pool.pop_back();
+    } else if (hipEventCreate(&pe.e[k]) != hipSuccess) {
+        pe.e[k] = nullptr;
+        return nullptr;
+    }
     hipEventRecord(pe.e[k], s);
     return pe.e[k];
 }
@@ -272,8 +290,9 @@ static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *tot
         set_error("hipHostMalloc of the readback slot failed");
         return TCR_ERR_HIP;
     }
-    TCR_HIP(hipMemcpyAsync(&rb->total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    TCR_HIP(hipMemcpyAsync(&rb->err, p.w.misc + kMiscErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, offsets + p.b.nq_total,
+                       (const uint32_t *)(p.w.misc + kMiscErr), rb);
+    TCR_LAUNCH_CHECK();
     TCR_HIP(hipStreamSynchronize(s));
     if (rb->err) {
         std::vector<CloudDev> dv(p.b.B);
@@ -440,6 +459,14 @@ int tcr_get_tuning(tcr_tuning *t) {
     return TCR_OK;
 }
 
+int tcr_csr_arrays(const int64_t *offsets, int64_t nq, int32_t *indptr, int64_t *row_entries, void *stream) {
+    if (!offsets || nq < 0) {
+        set_error("offsets missing or negative row count");
+        return TCR_ERR_ARG;
+    }
+    return csr_arrays_launch(offsets, nq, indptr, row_entries, (hipStream_t)stream);
+}
+
 int tcr_timing_enable(int32_t on) {
     g_timing.on = on != 0;
     return TCR_OK;
@@ -458,7 +485,11 @@ int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int
             ++k;
         }
         for (hipEvent_t e : pe.e)
-            if (e) hipEventDestroy(e);
+            if (e) {
+                // a recorded event is reusable once it has completed
+                if (hipEventSynchronize(e) == hipSuccess) g_timing.pool.push_back(e);
+                else hipEventDestroy(e);
+            }
     }
     g_timing.pending.clear();
     if (n_entries) *n_entries = k;
@@ -549,8 +580,9 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
         set_error("hipHostMalloc of the readback slot failed");
         return TCR_ERR_HIP;
     }
-    TCR_HIP(hipMemcpyAsync(&rb->total, offsets + p.b.nq_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    TCR_HIP(hipMemcpyAsync(&rb->err, p.w.misc + kMiscMlmFlags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, offsets + p.b.nq_total,
+                       (const uint32_t *)(p.w.misc + kMiscMlmFlags), rb);
+    TCR_LAUNCH_CHECK();
     TCR_HIP(hipStreamSynchronize(s));
     if (rb->err) {
         set_error(std::string("multi-level search does not take these inputs:") +

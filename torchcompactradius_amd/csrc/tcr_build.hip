@@ -80,9 +80,15 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
 // cloud in registers and LDS and issues one atomicMax per cloud it touched.
 __global__ void __launch_bounds__(kBlock) k_prep(CloudDev *__restrict__ desc, int32_t B,
                                                  const int64_t *__restrict__ q_start, int64_t nq, int nbq,
-                                                 const int64_t *__restrict__ r_start, int64_t nr, int nbr) {
+                                                 const int64_t *__restrict__ r_start, int64_t nr, int nbr,
+                                                 ClearSpans clear) {
     __shared__ uint32_t s_max[kBlock / kWave];
     __shared__ int s_cloud[kBlock / kWave];
+    // zero the sort and cell-extraction counters of this search (read only
+    // by kernels that run after this one)
+    for (int k = 0; k < ClearSpans::kMax; ++k)
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < clear.n[k]; i += (int64_t)gridDim.x * kBlock)
+            clear.p[k][i] = 0u;
     const bool qside = (int)blockIdx.x < nbq;
     const int bi = qside ? (int)blockIdx.x : (int)blockIdx.x - nbq;
     const int nb = qside ? nbq : nbr;
@@ -370,19 +376,6 @@ __global__ void k_cloud_cells(CloudDev *__restrict__ desc, int32_t B, const uint
 
 static inline unsigned blocks_for(int64_t n, int t = kBlock) { return (unsigned)((n + t - 1) / t); }
 
-// Empties every cloud's cell index: the whole open-addressing table, or the
-// first grid-cells int32 entries of a dense one.
-__global__ void __launch_bounds__(kBlock) k_clear_index(const CloudDev *__restrict__ desc, int32_t B,
-                                                        const int64_t *__restrict__ lt_start,
-                                                        uint64_t *__restrict__ lintab) {
-    for (int c = 0; c < B; ++c) {
-        const CloudDev &cd = desc[c];
-        const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
-        uint64_t *tab = lintab + lt_start[c];
-        for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < need; g += (int64_t)gridDim.x * kBlock)
-            tab[g] = ~0ull;
-    }
-}
 
 int build_prep(const Batch &b, Work &w, hipStream_t s) {
     // hMax + grid: one launch
@@ -391,8 +384,18 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
         if (!d.h.self_search && d.h.mode != TCR_MODE_GATHER && d.h.nr > 0) ref_side = true;
     const int nbq = (int)std::min<int64_t>(512, blocks_for(b.nq_total));
     const int nbr = ref_side ? (int)std::min<int64_t>(512, blocks_for(b.nr_total)) : 0;
+    ClearSpans clear{};
+    if (b.nr_total > 0 && b.rt_total > 0) {
+        clear.p[0] = w.sort_hist;
+        clear.n[0] = (int64_t)b.B * kMaxPasses * kRadix;
+        clear.p[1] = (uint32_t *)w.sort_lb;
+        clear.n[1] = 2 * (int64_t)kMaxPasses * b.rt_total * kRadix;
+        clear.p[2] = (uint32_t *)w.cell_lb;
+        clear.n[2] = 2 * b.rt_total;
+    }
     hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, b.B,
-                       (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr);
+                       (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
+                       clear);
     hipLaunchKernelGGL(k_grid, dim3((unsigned)std::min<int64_t>(64, (b.B + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        w.desc, b.B, w.misc);
     TCR_LAUNCH_CHECK();
@@ -402,15 +405,9 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
 int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
     const int64_t n = b.nr_total;
     if (n == 0) return TCR_OK;
+    // the sort's histogram kernel also empties the cell index; build_prep
+    // zeroed the look-back state of the sort and of k_cells
     int rc = radix_sort(b, w, s);
-    if (rc) return rc;
-    if (b.lt_total > 0) {
-        const int64_t blocks = std::min<int64_t>(2048, (b.lt_total + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_clear_index, dim3((unsigned)blocks), dim3(kBlock), 0, s, (const CloudDev *)w.desc, b.B,
-                           (const int64_t *)w.lt_start, w.lintab);
-        TCR_LAUNCH_CHECK();
-    }
-    rc = fill_u32(w.cell_lb, 0u, 2 * (size_t)b.rt_total, s);
     if (rc) return rc;
     CellsArgs ca;
     ca.desc = w.desc;

@@ -36,6 +36,8 @@
 // to the per-query kernel, as do all points of a cell whose first point is
 // outside the grid.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "tcr_internal.h"
 
@@ -76,6 +78,20 @@ __device__ __forceinline__ uint64_t bcast(uint64_t v, uint32_t m) {
     return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ int64_t bcast(int64_t v, uint32_t m) { return (int64_t)bcast((uint64_t)v, m); }
+// The backend's writelane intrinsic (no clang builtin for it here); the
+// backend sets up the lane select (m0) itself.
+extern "C" __device__ int tcr_writelane_i32(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+// v with lane m replaced by the uniform value x (v_writelane_b32: one VALU
+// operation, where a compare-and-select costs two moves and a select).
+__device__ __forceinline__ uint32_t put_lane(uint32_t v, uint32_t x, uint32_t m) {
+    return (uint32_t)tcr_writelane_i32((int)x, (int)m, (int)v);
+}
+// Position of this lane among the set lanes of uniform mask `bits`, plus base
+// (v_mbcnt_lo/hi).
+__device__ __forceinline__ int32_t lane_rank(uint64_t bits, int32_t base) {
+    return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(bits >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bits, (uint32_t)base));
+}
 
 // Decision thresholds on the estimated squared distance for "dist < h":
 // in if est < lo, out if est >= hi, exact test in between.
@@ -499,14 +515,14 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
             mt.lo -= mpi;
             mt.hi -= mpi;
         }
-        uint64_t mk[kSmallRounds];
+        // lane m collects query m's masks and count
+        uint32_t mk_lo[kSmallRounds], mk_hi[kSmallRounds];
 #pragma unroll
-        for (int r = 0; r < kSmallRounds; ++r) mk[r] = 0ull;
-        int32_t mhits = 0;
+        for (int r = 0; r < kSmallRounds; ++r) mk_lo[r] = mk_hi[r] = 0u;
+        uint32_t mhits = 0;
 
         for (uint32_t m = 0; m < nm; ++m) {
             if (!((todo >> m) & 1ull)) continue;
-            const bool mine = (uint32_t)lane == m;
             float xi2[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int d = 0; d < DIM; ++d) xi2[d] = bcast(mxc[d], m);
@@ -527,16 +543,17 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
                         const int32_t k = r * kWave + lane;
                         return k < v.total ? k + slot_off[k] : (int32_t)v.gb;
                     });
-                if (mine) mk[r] = hitm;
+                mk_lo[r] = put_lane(mk_lo[r], (uint32_t)hitm, m);
+                mk_hi[r] = put_lane(mk_hi[r], (uint32_t)(hitm >> 32), m);
                 hits += __popcll(hitm);
             }
-            if (mine) mhits = hits;
+            mhits = put_lane(mhits, (uint32_t)hits, m);
         }
         if (mvalid && !mfl) {
-            a.counts[mqg] = mhits;
+            a.counts[mqg] = (int32_t)mhits;
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
-                if (r < nrounds) a.masks[(int64_t)ms * kSmallRounds + r] = mk[r];
+                if (r < nrounds) a.masks[(int64_t)ms * kSmallRounds + r] = ((uint64_t)mk_hi[r] << 32) | mk_lo[r];
         }
     }
 }
@@ -586,7 +603,7 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
             for (int r = 0; r < kSmallRounds; ++r) {
                 if (r >= nrounds) break;
                 const uint64_t hitm = bcast(mk[r], m);
-                if ((hitm >> lane) & 1ull) rowbuf[len + __popcll(hitm & lanemask_lt())] = corig[r];
+                if (__builtin_amdgcn_inverse_ballot_w64(hitm)) rowbuf[lane_rank(hitm, len)] = corig[r];
                 len += __popcll(hitm);
             }
             __asm__ volatile("" ::: "memory");
@@ -773,13 +790,38 @@ __global__ void __launch_bounds__(256) k_sorted_offsets(const CloudDev *__restri
 
 // ------------------------------------------------------------- driver -----
 
-// Workgroups of the wavefront-per-cell kernels: a multiple of the 8 XCDs
-// (my_cells splits the cells by blockIdx % 8).
-static unsigned group_grid(const Batch &b) {
+// Workgroups of `kernel` that the current device holds at once (all CUs),
+// cached per kernel and device.
+static int64_t resident_workgroups(const void *kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int64_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(kernel, dev);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    int64_t r = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kGroupThreads, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+        r = (int64_t)per_cu * cus;
+    cache[key] = r;
+    return r;
+}
+
+// Workgroups of a wavefront-per-cell kernel: by default as many as the chip
+// holds at once, so that every wave starts at launch and walks an equal
+// share of the cells (a second, partial round of workgroups would run the
+// kernel's tail at low occupancy), and no more than one wave per reference
+// point (a bound on the occupied cells); a multiple of the 8 XCDs (my_cells splits the cells by
+// blockIdx % 8).
+static unsigned group_grid(const Batch &b, const void *kernel) {
     int64_t wgs = tuning().cell_workgroups;
     if (wgs <= 0) {
-        wgs = (b.nr_total + 3) / 4;   // upper bound on occupied cells / 4
-        if (wgs > 256 * 8) wgs = 256 * 8;
+        const int64_t resident = resident_workgroups(kernel);
+        wgs = resident > 0 ? (resident / 8) * 8 : 256 * 8;
+        wgs = std::min<int64_t>(wgs, (b.nr_total + 3) / 4);
     }
     wgs = ((wgs + 7) / 8) * 8;
     return (unsigned)(wgs > 0 ? wgs : 8);
@@ -790,17 +832,19 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
     bool seen[4][4] = {};
     for (const CloudDev &d : b.dev)
         if (d.h.self_search && d.h.search_range == 1 && d.h.nq > 0) seen[d.h.dim][d.h.mode] = true;
-    const dim3 grid(group_grid(b)), block(kGroupThreads), bgrid(256);
-#define TCR_L(D, M)                                                                          \
-    if (seen[D][M]) {                                                                        \
-        if (!FILL) hipLaunchKernelGGL((k_count_small<D, M>), grid, block, 0, s, ga);         \
-        hipLaunchKernelGGL((k_group_big<D, M, FILL>), bgrid, block, 0, s, ga);               \
+    const dim3 block(kGroupThreads), bgrid(256);
+#define TCR_L(D, M)                                                                                         \
+    if (seen[D][M]) {                                                                                       \
+        if (!FILL)                                                                                          \
+            hipLaunchKernelGGL((k_count_small<D, M>), dim3(group_grid(b, (const void *)k_count_small<D, M>)), \
+                               block, 0, s, ga);                                                            \
+        hipLaunchKernelGGL((k_group_big<D, M, FILL>), bgrid, block, 0, s, ga);                              \
     }
 #define TCR_LD(D) TCR_L(D, 0) TCR_L(D, 1) TCR_L(D, 2) TCR_L(D, 3)
     if (!FILL) {
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     } else {
-        hipLaunchKernelGGL(k_fill_small, grid, block, 0, s, ga);
+        hipLaunchKernelGGL(k_fill_small, dim3(group_grid(b, (const void *)k_fill_small)), block, 0, s, ga);
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     }
 #undef TCR_LD

@@ -157,6 +157,14 @@ size_t carve_work(const Batch &b, void *ws, Work &w);
 // Fills n 32-bit words with v (a kernel, so it is safe inside graph capture).
 int fill_u32(void *p, uint32_t v, size_t n, hipStream_t s);
 
+// Word ranges a kernel zeroes on the side (k_prep: the sort and cell
+// look-back state of the search).
+struct ClearSpans {
+    static constexpr int kMax = 3;
+    uint32_t *p[kMax];
+    int64_t n[kMax];
+};
+
 // Generic device-wide exclusive scan: out[i] = sum(in[0..i)), and
 // out[n] = total when write_total.  Tin: uint32_t or int32_t;
 // Tout: uint32_t or int64_t.
@@ -164,6 +172,7 @@ template <typename Tin, typename Tout>
 int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
                    hipStream_t s);
 size_t scan_partials_needed(int64_t n);
+int csr_arrays_launch(const int64_t *offsets, int64_t n, int32_t *indptr, int64_t *row_entries, hipStream_t s);
 
 // Launch-shape parameters (tcr_set_tuning); 0 = automatic.
 struct Tuning {

@@ -1,6 +1,8 @@
 // Device-wide exclusive scan (reduce -> scan partials -> apply), wave64.
 // Used for the offsets scan of the counts (cppSearch.py:81-82 restated with
 // int64 offsets), radix histograms, cell flags and hash buckets.
+#include <algorithm>
+
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -134,5 +136,26 @@ template int exclusive_scan<uint32_t, uint32_t>(const uint32_t *, uint32_t *, in
                                                 hipStream_t);
 template int exclusive_scan<int32_t, int64_t>(const int32_t *, int64_t *, int64_t, bool, uint64_t *,
                                               hipStream_t);
+
+// CSR arrays of the reference's SparseCSR from the int64 offsets
+// (neighborhood.py:303-305, util.py:335-358): indptr int32[n+1] and
+// rowEntries int64[n].
+__global__ void __launch_bounds__(kScanThreads) k_csr_arrays(const int64_t *__restrict__ offsets, int64_t n,
+                                                             int32_t *__restrict__ indptr,
+                                                             int64_t *__restrict__ row_entries) {
+    for (int64_t i = (int64_t)blockIdx.x * kScanThreads + threadIdx.x; i <= n; i += (int64_t)gridDim.x * kScanThreads) {
+        const int64_t o = offsets[i];
+        if (indptr) indptr[i] = (int32_t)o;
+        if (row_entries && i < n) row_entries[i] = offsets[i + 1] - o;
+    }
+}
+
+int csr_arrays_launch(const int64_t *offsets, int64_t n, int32_t *indptr, int64_t *row_entries, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(4096, (n + 1 + kScanThreads - 1) / kScanThreads);
+    hipLaunchKernelGGL(k_csr_arrays, dim3((unsigned)blocks), dim3(kScanThreads), 0, s, offsets, n, indptr,
+                       row_entries);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
 
 }  // namespace tcr

@@ -65,9 +65,21 @@ __device__ __forceinline__ uint32_t cloud_passes(const CloudDev &cd) {
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restrict__ t_start,
             const int64_t *__restrict__ p_start, const uint32_t *__restrict__ keys, uint32_t *__restrict__ hist,
-            uint32_t *__restrict__ misc) {
+            uint32_t *__restrict__ misc, const int64_t *__restrict__ lt_start, uint64_t *__restrict__ lintab) {
     __shared__ uint32_t h[kMaxPasses][kRadix];
     for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) (&h[0][0])[k] = 0;
+    // empty every cloud's cell index for k_cells (the irregular flag that
+    // picks the dense form is final once k_keys has run): the whole
+    // open-addressing table, or the first (grid cells + 1) / 2 words of a
+    // dense one
+    for (int c = 0; c < B; ++c) {
+        const CloudDev &cd = desc[c];
+        const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
+        uint64_t *tab = lintab + lt_start[c];
+        for (int64_t g = (int64_t)blockIdx.x * kSortThreads + threadIdx.x; g < need;
+             g += (int64_t)gridDim.x * kSortThreads)
+            tab[g] = ~0ull;
+    }
     __syncthreads();
     const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
     const CloudDev &cd = desc[ti.c];
@@ -222,13 +234,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
 
 int radix_sort(const Batch &b, Work &w, hipStream_t s) {
     if (b.rt_total == 0) return TCR_OK;
-    int rc = fill_u32(w.sort_hist, 0u, (size_t)b.B * kMaxPasses * kRadix, s);
-    if (rc) return rc;
-    rc = fill_u32(w.sort_lb, 0u, 2 * (size_t)kMaxPasses * b.rt_total * kRadix, s);
-    if (rc) return rc;
+    // sort_hist and sort_lb were zeroed by build_prep's k_prep
     hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, (const CloudDev *)w.desc,
                        b.B, (const int64_t *)w.rt_start, (const int64_t *)w.r_start, (const uint32_t *)w.keys0,
-                       w.sort_hist, w.misc);
+                       w.sort_hist, w.misc, (const int64_t *)w.lt_start, w.lintab);
     SortPassArgs a;
     a.desc = w.desc;
     a.B = b.B;

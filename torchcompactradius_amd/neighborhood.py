@@ -110,11 +110,13 @@ def radiusSearch(
             except Unsupported:
                 res = None      # the single-level search gives the same output
         if res is None:
-            res = run_search([spec], want_rows=(format == "coo"), index_dtype=indexDtype)
+            res = run_search([spec], want_rows=(format == "coo"), index_dtype=indexDtype, csr=(format == "csr"))
         nq = qpos.shape[0]
         nr = nq if rpos is None else rpos.shape[0]
         if format == "coo":
             out: Union[SparseCOO, SparseCSR] = SparseCOO(res.row, res.col, nq, nr)
+        elif res.indptr is not None:
+            out = SparseCSR(res.col, res.indptr, res.row_entries, nq, nr)
         else:
             out = SparseCSR(res.col, res.offsets.int(), res.offsets[1:] - res.offsets[:-1], nq, nr)
         if returnStructure:

@@ -107,6 +107,8 @@ class SearchResult(NamedTuple):
     total: int
     workspace: torch.Tensor
     clouds: ctypes.Array
+    indptr: Optional[torch.Tensor] = None       # int32 [nq_total + 1] (csr=True)
+    row_entries: Optional[torch.Tensor] = None  # int64 [nq_total] (csr=True)
 
 
 def _cloud_array(specs: Sequence[CloudSpec], q_offs, r_offs) -> ctypes.Array:
@@ -235,12 +237,21 @@ class SearchPlan:
                                          stream))
         return row, col
 
-    def run(self, capacity: Optional[int] = None) -> "SearchResult":
+    def _csr(self, stream):
+        indptr = torch.empty(self.nq + 1, dtype=torch.int32, device=self.device)
+        row_entries = torch.empty(self.nq, dtype=torch.int64, device=self.device)
+        check(self.L.tcr_csr_arrays(ptr(self.offsets), self.nq, ptr(indptr), ptr(row_entries), stream))
+        return indptr, row_entries
+
+    def run(self, capacity: Optional[int] = None, csr: bool = False) -> "SearchResult":
         """One search.  With a capacity guess (default: from the last search
         of this batch shape) count, scan and fill run back to back; a guess
-        that is too small costs one extra fill launch."""
+        that is too small costs one extra fill launch.  ``csr`` also returns
+        the int32 indptr and int64 rowEntries arrays, computed on the stream
+        before the host waits."""
         stream = _stream()
         self._install_tuning()
+        ip = re = None
         if capacity is None:
             hint = _pair_hint.get(self.key)
             capacity = None if hint is None else hint + hint // 32 + 256
@@ -248,18 +259,22 @@ class SearchPlan:
             row_buf, col_buf = self._alloc(capacity)
             check(self.L.tcr_search_launch(*self._args(), ptr(self.offsets), ptr(row_buf), ptr(col_buf),
                                            self.index_bytes, capacity, stream))
+            if csr:
+                ip, re = self._csr(stream)
             P = self._finish(stream)
             _remember(self.key, P)
             if P <= capacity:
                 return SearchResult(self.offsets, None if row_buf is None else row_buf[:P], col_buf[:P], P,
-                                    self.ws, self.clouds)
+                                    self.ws, self.clouds, ip, re)
         else:
             total = ctypes.c_int64(0)
             check(self.L.tcr_search_count(*self._args(), ptr(self.offsets), ctypes.byref(total), stream))
             P = int(total.value)
             _remember(self.key, P)
+            if csr:
+                ip, re = self._csr(stream)
         row, col = self._fill_exact(P, stream)
-        return SearchResult(self.offsets, row, col, P, self.ws, self.clouds)
+        return SearchResult(self.offsets, row, col, P, self.ws, self.clouds, ip, re)
 
     def capture(self, capacity: int) -> None:
         """Capture the whole search into a HIP graph with outputs of
@@ -304,9 +319,10 @@ class SearchPlan:
             pass
 
 
-def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.int64) -> "SearchResult":
+def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.int64,
+               csr: bool = False) -> "SearchResult":
     """Search a batch of clouds in one launch chain (see SearchPlan)."""
-    return SearchPlan(specs, want_rows, index_dtype).run()
+    return SearchPlan(specs, want_rows, index_dtype).run(csr=csr)
 
 
 def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_levels: int = 4) -> "SearchResult":
