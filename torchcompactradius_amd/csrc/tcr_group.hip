@@ -456,6 +456,45 @@ __device__ __forceinline__ uint64_t round_hits(const GroupArgs &a, const CloudDe
 
 // ------------------------------------------------------------ small cells --
 
+// Tests the member queries in `todo` (lane m = query mb + m) against the
+// cell's NR candidate rounds; lane m receives query m's hit masks.  NR is a
+// template argument so that the rounds are unrolled without a per-round
+// bound check.
+template <int DIM, int MODE, int NR>
+__device__ __forceinline__ void test_members(const GroupArgs &a, const CloudDev &cd, const Geo<DIM> &g,
+                                             const float (&cx)[kSmallRounds][3], const float (&cp)[kSmallRounds],
+                                             const float (&chh)[kSmallRounds], const Thr (&tj)[kSmallRounds],
+                                             const float (&mxc)[3], float mpi, float mqw, const Thr &mt,
+                                             uint64_t todo, uint32_t mb, const CellView &v,
+                                             const int32_t *slot_off, uint32_t (&mk_lo)[kSmallRounds],
+                                             uint32_t (&mk_hi)[kSmallRounds]) {
+    const int lane = __lane_id();
+    for (uint64_t rest = todo; rest; rest &= rest - 1) {
+        const uint32_t m = (uint32_t)__builtin_ctzll(rest);
+        float xi2[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xi2[d] = bcast(mxc[d], m);
+        float pi = 0.0f, hi = 0.0f;
+        Thr tq{0.f, 0.f};
+        if (MODE == TCR_MODE_GATHER) {
+            tq = Thr{bcast(mt.lo, m), bcast(mt.hi, m)};
+        } else {
+            pi = bcast(mpi, m);
+            hi = bcast(mqw, m);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const uint64_t hitm = round_hits<DIM, MODE>(
+                a, cd, g, cx[r], cp[r], chh[r], tj[r], xi2, pi, hi, tq, mb + m, [&]() {
+                    const int32_t k = r * kWave + lane;
+                    return k < v.total ? k + slot_off[k] : (int32_t)v.gb;
+                });
+            mk_lo[r] = put_lane(mk_lo[r], (uint32_t)hitm, m);
+            mk_hi[r] = put_lane(mk_hi[r], (uint32_t)(hitm >> 32), m);
+        }
+    }
+}
+
 template <int DIM, int MODE>
 __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid, int32_t *slot_off) {
     constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
@@ -469,13 +508,18 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
     float4 mq = a.sorted[ms0];
 
     const Geo<DIM> g(cd, v.center);
-    const int nrounds = (v.total + kWave - 1) / kWave;
+    const int nrounds = (v.total + kWave - 1) / kWave;  // 1..kSmallRounds
     expand_slots(v, slot_off, kOff);
 
     float cx[kSmallRounds][3], cp[kSmallRounds], chh[kSmallRounds];
     Thr tj[kSmallRounds];
 #pragma unroll
     for (int r = 0; r < kSmallRounds; ++r) {
+        cx[r][0] = cx[r][1] = cx[r][2] = 0.0f;
+        cp[r] = INFINITY;
+        chh[r] = 0.0f;
+        tj[r] = Thr{0.f, 0.f};
+        if (r >= nrounds) continue;  // uniform
         const int32_t k = r * kWave + lane;
         const bool valid = k < v.total;
         const int32_t j = valid ? k + slot_off[k] : (int32_t)v.gb;
@@ -515,41 +559,27 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
             mt.lo -= mpi;
             mt.hi -= mpi;
         }
-        // lane m collects query m's masks and count
+        // lane m collects query m's masks
         uint32_t mk_lo[kSmallRounds], mk_hi[kSmallRounds];
 #pragma unroll
         for (int r = 0; r < kSmallRounds; ++r) mk_lo[r] = mk_hi[r] = 0u;
-        uint32_t mhits = 0;
-
-        for (uint32_t m = 0; m < nm; ++m) {
-            if (!((todo >> m) & 1ull)) continue;
-            float xi2[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) xi2[d] = bcast(mxc[d], m);
-            float pi = 0.0f, hi = 0.0f;
-            Thr tq{0.f, 0.f};
-            if (MODE == TCR_MODE_GATHER) {
-                tq = Thr{bcast(mt.lo, m), bcast(mt.hi, m)};
-            } else {
-                pi = bcast(mpi, m);
-                hi = bcast(mq.w, m);
-            }
-            int32_t hits = 0;
-#pragma unroll
-            for (int r = 0; r < kSmallRounds; ++r) {
-                if (r >= nrounds) break;
-                const uint64_t hitm = round_hits<DIM, MODE>(
-                    a, cd, g, cx[r], cp[r], chh[r], tj[r], xi2, pi, hi, tq, mb + m, [&]() {
-                        const int32_t k = r * kWave + lane;
-                        return k < v.total ? k + slot_off[k] : (int32_t)v.gb;
-                    });
-                mk_lo[r] = put_lane(mk_lo[r], (uint32_t)hitm, m);
-                mk_hi[r] = put_lane(mk_hi[r], (uint32_t)(hitm >> 32), m);
-                hits += __popcll(hitm);
-            }
-            mhits = put_lane(mhits, (uint32_t)hits, m);
+#define TCR_TM(NR)                                                                                          \
+    case NR:                                                                                                \
+        test_members<DIM, MODE, NR>(a, cd, g, cx, cp, chh, tj, mxc, mpi, mq.w, mt, todo, mb, v, slot_off,  \
+                                    mk_lo, mk_hi);                                                          \
+        break;
+        switch (nrounds) {
+            TCR_TM(1) TCR_TM(2) TCR_TM(3) TCR_TM(4) TCR_TM(5) TCR_TM(6)
+            default: test_members<DIM, MODE, kSmallRounds>(a, cd, g, cx, cp, chh, tj, mxc, mpi, mq.w, mt, todo,
+                                                           mb, v, slot_off, mk_lo, mk_hi);
         }
+#undef TCR_TM
         if (mvalid && !mfl) {
+            // the count of a query is the population of its masks (rounds
+            // past nrounds hold zero)
+            uint32_t mhits = 0;
+#pragma unroll
+            for (int r = 0; r < kSmallRounds; ++r) mhits += __popc(mk_lo[r]) + __popc(mk_hi[r]);
             a.counts[mqg] = (int32_t)mhits;
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
