@@ -7,8 +7,9 @@ inputs resident in HBM; the step runs radiusSearch(..., algorithm='compact',
 format='csr') end to end: structure build, count, scan and fill (into
 outputs sized from the previous step's total, so the chain runs without a
 host round trip in between), one host sync for the pair total, and the CSR
-tuple.  The roofline figure is the fill pass, timed with HIP events that the
-library records on the search stream during the timed steps.
+tuple.  The roofline figure is the fill kernel (k_fill_small), timed with
+HIP events that the library records on the search stream around it during
+the timed steps.
 
 Prints one JSON line (rank 0).  The CPU baseline is the C/OpenMP oracle
 timed on this host on the same workload.
@@ -40,8 +41,8 @@ def algorithmic_bytes(nq: int, dim: int, pairs: int, support_read: bool = True) 
 
 
 def fill_bytes(nq: int, dim: int, pairs: int) -> int:
-    """Algorithmic bytes of the fill pass: positions + supports read, int64
-    offsets read, int64 column indices written."""
+    """Algorithmic bytes of the fill kernel: positions + supports read, int64
+    row offsets read, int64 column indices written."""
     return nq * (4 * dim + 4) + 8 * (nq + 1) + pairs * 8
 
 
@@ -49,12 +50,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_fill_periodic3d_1M.md")
 
 
 def measured_traffic():
-    """HBM bytes per fill pass from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (line 'traffic_bytes_per_search: N'), or None."""
+    """HBM bytes per fill-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (line 'traffic_bytes_per_launch: N'), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_search:"):
+                if line.startswith("traffic_bytes_per_launch:"):
                     return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -298,7 +299,7 @@ def main():
     P = int(out.indices.numel())
 
     # the library records HIP events on the search stream around the count
-    # phase and the fill of every search of the timed region
+    # phase and around the fill kernel of every search of the timed region
     L = _lib.load()
     L.tcr_timing_collect(None, None, 0, None)
     L.tcr_timing_enable(1)
@@ -340,12 +341,12 @@ def main():
         "data": "synthetic (seeded jittered grid)",
         "config": {"workload": w.name, "points": nq, "dim": 3, "periodic": "xyz", "mode": w.mode,
                    "format": "csr", "hash_map_length": w.hash_map_length, "pairs": P},
-        "roofline": {"bound": "hbm", "kernel": "fill pass", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_fill_small", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "algorithmic_bytes": fb, "avg_ms": fill_ms, "launches": len(fill)},
         "end_to_end": {"algorithmic_bytes": e2e_bytes, "GB/s": e2e_bytes / (ms / 1e3) / 1e9,
                        "frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "count_phase_ms": count_ms,
-                       "fill_phase_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},
+                       "fill_kernel_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},
     }
     if not args.no_batched:
         line["batched"] = bench_batched(torch)

@@ -219,8 +219,10 @@ int tcr_csr_arrays(const int64_t *offsets, int64_t nq, int32_t *indptr, int64_t 
 
 /* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
  * (no counterpart in the reference; a benchmark hook).  While enabled, each
- * launch records HIP events on its stream around the count phase (build,
- * count, scan) and the fill.  tcr_timing_collect waits for the recorded
+ * launch records HIP events on its stream that delimit the count phase
+ * (build, count, scan, up to the fill kernel) and the fill pass's main kernel
+ * (the wavefront-per-cell fill, else the per-query fill).  tcr_timing_collect
+ * waits for the recorded
  * events, writes up to max_entries milliseconds per launch (-1 for a phase
  * that did not run), the number written to *n_entries, and forgets them. */
 int tcr_timing_enable(int32_t on);

@@ -236,17 +236,25 @@ struct Timing {
 };
 static thread_local Timing g_timing;
 
-static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
+static hipEvent_t pooled_event() {
+    hipEvent_t e = nullptr;
     if (!g_timing.pool.empty()) {
-        pe.e[k] = g_timing.pool.back();
+        e = g_timing.pool.back();
         g_timing.pool.pop_back();
-    } else if (hipEventCreate(&pe.e[k]) != hipSuccess) {
-        pe.e[k] = nullptr;
-        return nullptr;
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
     }
-    hipEventRecord(pe.e[k], s);
+    return e;
+}
+
+static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
+    pe.e[k] = pooled_event();
+    if (pe.e[k]) hipEventRecord(pe.e[k], s);
     return pe.e[k];
 }
+
+static thread_local KernelEvents g_fill_events;
+KernelEvents &fill_events() { return g_fill_events; }
 
 // Everything of a search up to the scan, and the fill when capacity >= 0
 // (skipped on the device if the total exceeds it).  Asynchronous.
@@ -268,16 +276,31 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     if (rc) return rc;
     rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
     if (rc) return rc;
-    if (timed) timing_event(pe, 1, s);
     if (capacity >= 0) {
         if (!out.col) {
             set_error("col missing");
             return TCR_ERR_ARG;
         }
-        if (timed) timing_event(pe, 2, s);
+        // events 1 and 3 bracket the fill pass's main kernel (recorded by
+        // the launcher: the wavefront-per-cell fill, else the per-query
+        // fill); the count phase ends at event 1
+        KernelEvents &ke = fill_events();
+        ke = KernelEvents{};
+        if (timed) {
+            pe.e[1] = pooled_event();
+            pe.e[3] = pooled_event();
+            ke.before = pe.e[1];
+            ke.after = pe.e[3];
+        }
         rc = query_fill(p.b, p.w, offsets, out, capacity, s);
-        if (timed) timing_event(pe, 3, s);
+        if (timed && !ke.recorded) {  // no fill kernel ran
+            if (pe.e[1]) hipEventRecord(pe.e[1], s);
+            if (pe.e[3]) hipEventRecord(pe.e[3], s);
+        }
+        ke = KernelEvents{};
         pe.has_fill = true;
+    } else if (timed) {
+        timing_event(pe, 1, s);
     }
     if (timed) g_timing.pending.push_back(pe);
     return rc;
@@ -478,8 +501,8 @@ int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int
         if (k < max_entries) {
             float c = -1.0f, f = -1.0f;
             if (pe.e[1] && hipEventSynchronize(pe.e[1]) == hipSuccess) hipEventElapsedTime(&c, pe.e[0], pe.e[1]);
-            if (pe.has_fill && pe.e[3] && hipEventSynchronize(pe.e[3]) == hipSuccess)
-                hipEventElapsedTime(&f, pe.e[2], pe.e[3]);
+            if (pe.has_fill && pe.e[1] && pe.e[3] && hipEventSynchronize(pe.e[3]) == hipSuccess)
+                hipEventElapsedTime(&f, pe.e[1], pe.e[3]);
             if (count_ms) count_ms[k] = c;
             if (fill_ms) fill_ms[k] = f;
             ++k;

@@ -874,7 +874,9 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
     if (!FILL) {
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     } else {
+        fill_kernel_begin(s);
         hipLaunchKernelGGL(k_fill_small, dim3(group_grid(b, (const void *)k_fill_small)), block, 0, s, ga);
+        fill_kernel_end(s);
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     }
 #undef TCR_LD

@@ -157,6 +157,26 @@ size_t carve_work(const Batch &b, void *ws, Work &w);
 // Fills n 32-bit words with v (a kernel, so it is safe inside graph capture).
 int fill_u32(void *p, uint32_t v, size_t n, hipStream_t s);
 
+// Optional HIP events around the fill pass's main kernel (benchmark timing):
+// the launcher records `before` and `after` around the first fill kernel it
+// launches and sets `recorded`.
+struct KernelEvents {
+    hipEvent_t before = nullptr, after = nullptr;
+    bool recorded = false;
+};
+KernelEvents &fill_events();  // this thread's
+inline void fill_kernel_begin(hipStream_t s) {
+    KernelEvents &k = fill_events();
+    if (k.before && !k.recorded) hipEventRecord(k.before, s);
+}
+inline void fill_kernel_end(hipStream_t s) {
+    KernelEvents &k = fill_events();
+    if (k.after && !k.recorded) {
+        hipEventRecord(k.after, s);
+        k.recorded = true;
+    }
+}
+
 // Word ranges a kernel zeroes on the side (k_prep: the sort and cell
 // look-back state of the search).
 struct ClearSpans {

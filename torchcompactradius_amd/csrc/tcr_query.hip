@@ -243,8 +243,13 @@ int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out,
     } else {
         if (b.nq_total == 0) return TCR_OK;
         const dim3 grid((unsigned)((b.nq_total + kQBlock - 1) / kQBlock)), block(kQBlock);
-        if (fill) hipLaunchKernelGGL(k_query_generic<true>, grid, block, 0, s, a);
-        else hipLaunchKernelGGL(k_query_generic<false>, grid, block, 0, s, a);
+        if (fill) {
+            fill_kernel_begin(s);
+            hipLaunchKernelGGL(k_query_generic<true>, grid, block, 0, s, a);
+            fill_kernel_end(s);
+        } else {
+            hipLaunchKernelGGL(k_query_generic<false>, grid, block, 0, s, a);
+        }
     }
     TCR_LAUNCH_CHECK();
     return TCR_OK;
