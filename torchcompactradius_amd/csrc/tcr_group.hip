@@ -591,6 +591,21 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
 // Replays the stored masks.  Each query's row is assembled in LDS (rows of
 // a small cell hold at most kSmallChunk entries) and leaves in coalesced
 // stores.  Independent of mode and dimension except for the stencil size.
+// Output kinds of the fill kernel: bit 1 = int32 indices (else int64),
+// bit 0 = row indices written too (COO).
+template <int KIND>
+__device__ __forceinline__ void put_pair(const PairOut &o, int64_t pos, int64_t r, int32_t c) {
+    if (KIND & 2) {
+        ((int32_t *)o.col)[pos] = c;
+        if (KIND & 1) ((int32_t *)o.row)[pos] = (int32_t)r;
+    } else {
+        ((int64_t *)o.col)[pos] = c;
+        if (KIND & 1) ((int64_t *)o.row)[pos] = r;
+    }
+}
+static int pair_kind(const PairOut &o) { return (o.idx32 ? 2 : 0) | (o.row ? 1 : 0); }
+
+template <int KIND>
 __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int32_t *slot_off, int32_t *rowbuf) {
     const int lane = __lane_id();
     const int nrounds = (v.total + kWave - 1) / kWave;
@@ -639,7 +654,9 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
             __asm__ volatile("" ::: "memory");
             const int64_t rbase = bcast(mbase, m);
             const int64_t qloc = (int64_t)bcast(mloc, m);
-            for (int32_t k = lane; k < len; k += kWave) a.out.put(rbase + k, qloc, rowbuf[k]);
+            if (lane < len) put_pair<KIND>(a.out, rbase + lane, qloc, rowbuf[lane]);
+            for (int32_t k = lane + kWave; k < len; k += kWave)  // rows longer than a wavefront
+                put_pair<KIND>(a.out, rbase + k, qloc, rowbuf[k]);
             __asm__ volatile("" ::: "memory");
         }
     }
@@ -767,6 +784,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
     }
 }
 
+template <int KIND>
 __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
@@ -784,7 +802,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
             const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
             const CellView v = make_view(a, cur, noff);
             if (v.total <= kSmallChunk)  // bigger cells: k_group_big
-                fill_small(a, v, noff, s_slot[threadIdx.x / kWave], s_row[threadIdx.x / kWave]);
+                fill_small<KIND>(a, v, noff, s_slot[threadIdx.x / kWave], s_row[threadIdx.x / kWave]);
         }
         cur = nxt;
     }
@@ -875,7 +893,12 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     } else {
         fill_kernel_begin(s);
-        hipLaunchKernelGGL(k_fill_small, dim3(group_grid(b, (const void *)k_fill_small)), block, 0, s, ga);
+        switch (pair_kind(ga.out)) {
+            case 0: hipLaunchKernelGGL(k_fill_small<0>, dim3(group_grid(b, (const void *)k_fill_small<0>)), block, 0, s, ga); break;
+            case 1: hipLaunchKernelGGL(k_fill_small<1>, dim3(group_grid(b, (const void *)k_fill_small<1>)), block, 0, s, ga); break;
+            case 2: hipLaunchKernelGGL(k_fill_small<2>, dim3(group_grid(b, (const void *)k_fill_small<2>)), block, 0, s, ga); break;
+            default: hipLaunchKernelGGL(k_fill_small<3>, dim3(group_grid(b, (const void *)k_fill_small<3>)), block, 0, s, ga); break;
+        }
         fill_kernel_end(s);
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
     }
