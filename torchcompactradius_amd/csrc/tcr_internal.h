@@ -167,12 +167,12 @@ struct KernelEvents {
 KernelEvents &fill_events();  // this thread's
 inline void fill_kernel_begin(hipStream_t s) {
     KernelEvents &k = fill_events();
-    if (k.before && !k.recorded) hipEventRecord(k.before, s);
+    if (k.before && !k.recorded) (void)hipEventRecord(k.before, s);
 }
 inline void fill_kernel_end(hipStream_t s) {
     KernelEvents &k = fill_events();
     if (k.after && !k.recorded) {
-        hipEventRecord(k.after, s);
+        (void)hipEventRecord(k.after, s);
         k.recorded = true;
     }
 }
