@@ -648,7 +648,9 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
             for (int r = 0; r < kSmallRounds; ++r) {
                 if (r >= nrounds) break;
                 const uint64_t hitm = bcast(mk[r], m);
-                if (__builtin_amdgcn_inverse_ballot_w64(hitm)) rowbuf[lane_rank(hitm, len)] = corig[r];
+                // uniform base (SGPR) + lane rank: one shift-add for the address
+                int32_t *const rowr = rowbuf + len;
+                if (__builtin_amdgcn_inverse_ballot_w64(hitm)) rowr[lane_rank(hitm, 0)] = corig[r];
                 len += __popcll(hitm);
             }
             __asm__ volatile("" ::: "memory");
@@ -802,7 +804,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
             const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
             const CellView v = make_view(a, cur, noff);
             if (v.total <= kSmallChunk)  // bigger cells: k_group_big
-                fill_small<KIND>(a, v, noff, s_slot[threadIdx.x / kWave], s_row[threadIdx.x / kWave]);
+                fill_small<KIND>(a, v, noff, s_slot[threadIdx.x / kWave], s_row[uniform(threadIdx.x / kWave)]);
         }
         cur = nxt;
     }
