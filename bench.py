@@ -71,23 +71,47 @@ def copy_probe(torch):
     torch.cuda.synchronize()
 
 
-def cpu_baseline(w, reps: int = 3):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, reps: int = 3, single=None):
+    """The C/OpenMP oracle on this host: the full workload on all cores the
+    process may use (at most 16), median of `reps`; plus one single-thread
+    search of `single` (a smaller cloud of the same density, so that the
+    single-thread leg stays a few seconds)."""
     import oracle
     oracle.build()
-    threads = min(os.cpu_count() or 1, 16)
+    threads = max(1, min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1),
+                         int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+
+    def one(cloud, nthreads):
+        t0 = time.perf_counter()
+        r = oracle.radius_search(cloud.pos, cloud.sup, domain_min=cloud.domain_min, domain_max=cloud.domain_max,
+                                 periodic=cloud.periodic, mode=cloud.mode, hash_map_length=cloud.hash_map_length,
+                                 want_rows=False, threads=nthreads)
+        return time.perf_counter() - t0, int(r.col.shape[0])
+
     ts = []
     P = 0
     for _ in range(reps):
-        t0 = time.perf_counter()
-        r = oracle.radius_search(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max,
-                                 periodic=w.periodic, mode=w.mode, hash_map_length=w.hash_map_length,
-                                 want_rows=False, threads=threads)
-        ts.append(time.perf_counter() - t0)
-        P = r.col.shape[0]
+        dt, P = one(w, threads)
+        ts.append(dt)
     t = statistics.median(ts)
+    s1 = single if single is not None else w
+    t1, P1 = one(s1, 1)
     return {"value": P / t, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"full workload ({w.pos.shape[0]} points, {P} pairs), median of {reps} searches, "
-                      f"{t * 1e3:.0f} ms each"}
+                      f"{t * 1e3:.0f} ms each; single thread: {s1.pos.shape[0]} points of the same density, "
+                      f"{P1} pairs, {t1 * 1e3:.0f} ms",
+            "single_thread_value": P1 / t1, "cpu_model": _cpu_model()}
 
 
 def bench_batched(torch, reps: int = 5, scale: float = 1.0):
@@ -355,7 +379,7 @@ def main():
     if not args.no_adaptive:
         line["adaptive"] = bench_adaptive(torch)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w)
+        line["cpu_baseline"] = cpu_baseline(w, single=synthetic.periodic_3d(n=max(10, args.n // 2)))
     print(json.dumps(line), flush=True)
 
 
