@@ -128,7 +128,7 @@ struct __align__(16) CellRec {
     uint32_t gb, ge;    // sorted slots [gb, ge) of the cell's points
     uint32_t cloud;
     uint32_t state;     // kCellSkip / kCellFast / kCellStray
-    float4 center;
+    float4 center;      // w: 1 if a stencil neighbour lies across a periodic boundary, else 0
 };
 
 struct GroupArgs {
@@ -258,8 +258,13 @@ __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
                                                                              : cell_coords<3>(cd, p0, q);
         const bool fast = inrange && cd.fast;
         float cen[3] = {0.f, 0.f, 0.f};
-        for (int d = 0; d < dim; ++d) cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
-        a.recs[cid] = CellRec{gb, ge, c, fast ? kCellFast : kCellStray, make_float4(cen[0], cen[1], cen[2], 0.0f)};
+        bool wrap = false;  // a periodic neighbour coordinate wraps around
+        for (int d = 0; d < dim; ++d) {
+            cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
+            if (((cd.h.periodic_mask >> d) & 1u) && (q[d] == 0 || q[d] == cd.nc[d] - 1)) wrap = true;
+        }
+        a.recs[cid] = CellRec{gb, ge, c, fast ? kCellFast : kCellStray,
+                              make_float4(cen[0], cen[1], cen[2], wrap ? 1.0f : 0.0f)};
         if (!fast) {
             for (int o = 0; o < kStencil; ++o) out[o] = make_int2(0, 0);
             continue;
@@ -278,6 +283,7 @@ __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
 // cell o's first global slot and exclusive candidate prefix.
 struct CellView {
     uint32_t c, gb, ge, state;
+    bool wrap;           // some stencil candidates are periodic images
     float4 center;
     int64_t qs;          // first global query index of the cloud
     int32_t rbeg, pre;   // per lane (stencil cell `lane`)
@@ -304,6 +310,7 @@ __device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad
     v.c = uniform(l.rec.cloud);
     v.state = uniform(l.rec.state);
     v.center = make_float4(uniform_f(l.rec.center.x), uniform_f(l.rec.center.y), uniform_f(l.rec.center.z), 0.f);
+    v.wrap = uniform_f(l.rec.center.w) != 0.0f;
     v.qs = uniform(a.q_start[v.c]);
     const int32_t cnt = lane < noff ? l.rng.y : 0;
     int32_t incl = cnt;
@@ -393,6 +400,25 @@ struct Geo {
                 else if (e0 < -0.5f * ext[d]) x = x + ext[d];
             }
             const float c = x - center[d];
+            well = well && fabsf(c) <= box;
+            xc[d] = c;
+            p = d == 0 ? c * c : fmaf(c, c, p);
+        }
+        if (!valid) p = INFINITY;
+        else if (!well) p = __int_as_float(0x7fc00000);
+    }
+    // place() for a cell none of whose stencil neighbours lies across a
+    // periodic boundary: a candidate of such a stencil is within 1.5 cells of
+    // the centre and a periodic extent spans >= 6 cells, so no image shift
+    // applies (an aliased candidate from elsewhere fails the box test as in
+    // place() and is decided exactly)
+    __device__ __forceinline__ void place_direct(float4 v, bool valid, float *xc, float &p) const {
+        const float xv[3] = {v.x, v.y, v.z};
+        bool well = true;
+        p = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const float c = xv[d] - center[d];
             well = well && fabsf(c) <= box;
             xc[d] = c;
             p = d == 0 ? c * c : fmaf(c, c, p);
@@ -524,7 +550,8 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
         const bool valid = k < v.total;
         const int32_t j = valid ? k + slot_off[k] : (int32_t)v.gb;
         const float4 cv = a.sorted[j];
-        g.place(cv, valid, cx[r], cp[r]);
+        if (v.wrap) g.place(cv, valid, cx[r], cp[r]);
+        else g.place_direct(cv, valid, cx[r], cp[r]);
         chh[r] = cv.w;
         if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
     }
