@@ -350,6 +350,32 @@ def main():
     if args.copy_probe:
         copy_probe(torch)
 
+    # the same search with the int32-index option (an extension: the
+    # reference returns int64), same timing
+    def step32():
+        return radiusSearch(pc, None, mode=w.mode, domain=dom, hashMapLength=w.hash_map_length,
+                            algorithm="compact", format="csr", indexDtype=torch.int32)
+
+    for _ in range(args.warmup):
+        step32()
+    torch.cuda.synchronize()
+    L.tcr_timing_collect(None, None, 0, None)
+    L.tcr_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step32()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.tcr_timing_enable(0)
+    got32 = ctypes.c_int32(0)
+    L.tcr_timing_collect(cms, fms, args.steps, ctypes.byref(got32))
+    fill32 = [x for x in fms[:got32.value] if x >= 0]
+    ms32 = (t1 - t0) * 1e3 / args.steps
+    fill32_ms = statistics.mean(fill32) if fill32 else float("nan")
+    fb32 = nq * 16 + 8 * (nq + 1) + 4 * P
+    int32_line = {"ms_per_step": ms32, "pairs_per_s": P / (ms32 / 1e3), "fill_kernel_ms": fill32_ms,
+                  "fill_GB/s": fb32 / (fill32_ms / 1e3) / 1e9, "fill_algorithmic_bytes": fb32}
+
     line = {
         "metric": METRIC,
         "value": pairs_per_s,
@@ -371,6 +397,7 @@ def main():
         "end_to_end": {"algorithmic_bytes": e2e_bytes, "GB/s": e2e_bytes / (ms / 1e3) / 1e9,
                        "frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "count_phase_ms": count_ms,
                        "fill_kernel_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},
+        "int32_indices": int32_line,
     }
     if not args.no_batched:
         line["batched"] = bench_batched(torch)
