@@ -788,8 +788,10 @@ __device__ __forceinline__ bool cloud_is(const CloudDev &cd, int dim, int mode) 
     return cd.h.dim == dim && (mode < 0 || cd.h.mode == mode);
 }
 
+// 5 waves per SIMD: the register budget (<= 102 VGPRs) spills only per-cell
+// state, never the query loop
 template <int DIM, int MODE>
-__global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
+__global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) k_count_small(GroupArgs a) {
     constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     const CellRange cr = my_cells(a.misc[kMiscCells]);
@@ -813,8 +815,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_count_small(GroupArgs a) {
     }
 }
 
+// 8 waves per SIMD (<= 64 VGPRs, no spills)
 template <int KIND>
-__global__ void __launch_bounds__(kGroupThreads) k_fill_small(GroupArgs a) {
+__global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) k_fill_small(GroupArgs a) {
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
     if (*a.total > a.capacity) return;
