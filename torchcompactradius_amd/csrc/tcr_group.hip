@@ -796,12 +796,10 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
     const CellRange cr = my_cells(a.misc[kMiscCells]);
     if (cr.first >= cr.end) return;
-    CellLoad cur = load_cell(a, cr.first);
+    // no next-cell prefetch here: its registers would cost the occupancy
+    // that hides the same latency
     for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
-        const uint32_t nid = cid + cr.step;
-        CellLoad nxt = cur;
-        if (nid < cr.end) nxt = load_cell(a, nid);  // in flight during this cell
-        const CellView v = make_view(a, cur, kOff);
+        const CellView v = make_view(a, load_cell(a, cid), kOff);
         if (v.state != kCellSkip && cloud_is(a.desc[v.c], DIM, MODE)) {
             if (v.state == kCellStray) {
                 append_strays(a, v);
@@ -811,7 +809,6 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
                 count_small<DIM, MODE>(a, v, cid, s_slot[threadIdx.x / kWave]);
             }
         }
-        cur = nxt;
     }
 }
 
