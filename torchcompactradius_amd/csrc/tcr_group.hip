@@ -820,11 +820,8 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     if (*a.total > a.capacity) return;
     const CellRange cr = my_cells(a.misc[kMiscCells]);
     if (cr.first >= cr.end) return;
-    CellLoad cur = load_cell(a, cr.first);
     for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
-        const uint32_t nid = cid + cr.step;
-        CellLoad nxt = cur;
-        if (nid < cr.end) nxt = load_cell(a, nid);
+        const CellLoad cur = load_cell(a, cid);
         const uint32_t state = uniform(cur.rec.state);
         if (state == kCellFast) {
             const int dim = a.desc[uniform(cur.rec.cloud)].h.dim;
@@ -833,7 +830,6 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
             if (v.total <= kSmallChunk)  // bigger cells: k_group_big
                 fill_small<KIND>(a, v, noff, s_slot[threadIdx.x / kWave], s_row[uniform(threadIdx.x / kWave)]);
         }
-        cur = nxt;
     }
 }
 
