@@ -145,7 +145,8 @@ struct GroupArgs {
     const uint8_t *sflags;       // per sorted slot: nonzero = stray query
     int2 *ranges;                // [cells * 27] (first global slot, count)
     CellRec *recs;               // [cells]
-    uint64_t *masks;             // [nr * kSmallRounds]
+    uint64_t *masks;             // [kSmallRounds][nr]: round-major, so a batch's loads and stores coalesce
+    int64_t mask_stride;         // nr (reference points of the batch)
     int32_t *counts;             // [nq] per global query index
     const int64_t *offsets;      // [nq+1] (fill)
     const int64_t *total;        // offsets + nq: the fill runs only if *total <= capacity
@@ -610,7 +611,7 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
             a.counts[mqg] = (int32_t)mhits;
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
-                if (r < nrounds) a.masks[(int64_t)ms * kSmallRounds + r] = ((uint64_t)mk_hi[r] << 32) | mk_lo[r];
+                if (r < nrounds) a.masks[r * a.mask_stride + ms] = ((uint64_t)mk_hi[r] << 32) | mk_lo[r];
         }
     }
 }
@@ -643,7 +644,7 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
     uint64_t mk[kSmallRounds];
 #pragma unroll
     for (int r = 0; r < kSmallRounds; ++r)
-        mk[r] = (r < nrounds && live) ? a.masks[(int64_t)ms0 * kSmallRounds + r] : 0ull;
+        mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms0] : 0ull;
     int64_t mbase = live ? a.soff[ms0] : 0;
     int32_t mloc = a.sort_index[ms0];
 
@@ -662,7 +663,7 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
             live = mvalid && !a.sflags[ms];
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
-                mk[r] = (r < nrounds && live) ? a.masks[(int64_t)ms * kSmallRounds + r] : 0ull;
+                mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms] : 0ull;
             mbase = live ? a.soff[ms] : 0;
             mloc = a.sort_index[ms];
         }
@@ -952,6 +953,7 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
     ga.ranges = w.ranges;
     ga.recs = (CellRec *)w.recs;
     ga.masks = w.masks;
+    ga.mask_stride = b.nr_total;
     ga.counts = w.counts;
     ga.offsets = offsets;
     ga.total = offsets ? offsets + b.nq_total : nullptr;
