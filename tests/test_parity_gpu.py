@@ -142,6 +142,29 @@ def test_small_periodic_grids_visit_cells_twice():
     assert_same(g, o)
 
 
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_clustered_big_cells_and_member_batches(mode, periodic):
+    """Dense clusters: cells whose stencils hold more candidates than one
+    register chunk (the chunked big-cell kernel), a cell of more than one
+    wavefront of points with empty neighbours (several member batches in the
+    small-cell kernels), and sparse background points."""
+    rng = np.random.default_rng(21 + len(mode) + (7 if periodic else 0))
+    bg = rng.uniform(-1, 1, (2000, 3))
+    ball = rng.normal(0.3, 0.03, (1500, 3))
+    lone = np.array([-0.75, -0.75, 0.75]) + rng.uniform(-0.03, 0.03, (100, 3))
+    q = np.concatenate([bg, ball, lone]).astype(np.float32)
+    q = np.clip(q, -0.999, 0.999)
+    h = rng.uniform(0.08, 0.12, q.shape[0]).astype(np.float32)
+    per = [periodic] * 3
+    print(f"case: clustered mode={mode} periodic={periodic}", flush=True)
+    g = gpu_search(q, h, mn=[-1] * 3, mx=[1] * 3, periodic=per, mode=mode, L=4096)
+    o = oracle.radius_search(q, h, domain_min=[-1] * 3, domain_max=[1] * 3, periodic=per, mode=mode,
+                             hash_map_length=4096)
+    assert_same(g, o)
+    assert o.counts.max() > 448  # the ball's rows exceed one register chunk
+
+
 def test_empty_and_tiny():
     rng = np.random.default_rng(4)
     r = rng.uniform(-1, 1, (64, 3)).astype(np.float32)
