@@ -696,6 +696,10 @@ int tcr_search_export(const tcr_cloud *clouds, int32_t n_clouds, int32_t cloud, 
     const CloudDev &hd = p.b.dev[cloud];
     int64_t r0 = 0;
     for (int32_t c = 0; c < cloud; ++c) r0 += p.b.dev[c].h.nr;
+    if (p.b.nr_total > 0) {
+        rc = cloud_cells_launch(p.b, p.w, s);  // per-cloud cell counts (the search skips them)
+        if (rc) return rc;
+    }
     const int64_t nr = hd.h.nr;
     const unsigned g = (unsigned)((nr + 255) / 256);
     if (nr > 0) {

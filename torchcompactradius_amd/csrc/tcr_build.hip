@@ -426,6 +426,13 @@ int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
     ca.lb = w.cell_lb;
     ca.misc = w.misc;
     hipLaunchKernelGGL(k_cells, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, ca);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
+
+// Per-cloud cell counts (desc[c].ncells) -- read only by the structure
+// export, which launches this first.
+int cloud_cells_launch(const Batch &b, Work &w, hipStream_t s) {
     hipLaunchKernelGGL(k_cloud_cells, dim3(1), dim3(64), 0, s, w.desc, b.B, (const uint32_t *)w.misc);
     TCR_LAUNCH_CHECK();
     return TCR_OK;

@@ -226,6 +226,7 @@ int build_structure(const Batch &b, Work &w, hipStream_t s);
 // (keys0/vals0/rwrap/pflags) are written -- sort, cells and lin table.
 int build_prep(const Batch &b, Work &w, hipStream_t s);
 int build_sorted_cells(const Batch &b, Work &w, hipStream_t s);
+int cloud_cells_launch(const Batch &b, Work &w, hipStream_t s);
 
 // Multi-level search of one cloud (tcr_mlm.hip): level keys, group table,
 // count/fill traversal, per-row ordering.
