@@ -10,6 +10,7 @@ Reference counterpart: neighborSearch_cpp + searchNeighbors_cpp
 from __future__ import annotations
 
 import ctypes
+from collections import OrderedDict
 from dataclasses import dataclass
 from typing import List, NamedTuple, Optional, Sequence, Tuple
 
@@ -319,10 +320,39 @@ class SearchPlan:
             pass
 
 
+# Plans of recent single-cloud searches, keyed by the input buffers, the
+# cloud description, the output kind and the stream: a repeated search of the
+# same tensors (the usual simulation loop, positions updated in place) reuses
+# the plan's descriptor array, workspace and offsets instead of rebuilding
+# them.  A plan keeps its input tensors alive, so a key's data pointers
+# cannot be taken by other tensors while it is cached.  Nothing a search
+# returns to the caller aliases the plan's buffers.
+_plan_cache: "OrderedDict[tuple, SearchPlan]" = OrderedDict()
+_PLAN_CACHE_MAX = 4
+
+
+def _tensor_key(t: Optional[torch.Tensor]):
+    return None if t is None else (t.data_ptr(), tuple(t.shape), t.stride(), t.dtype)
+
+
 def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.int64,
                csr: bool = False) -> "SearchResult":
     """Search a batch of clouds in one launch chain (see SearchPlan)."""
-    return SearchPlan(specs, want_rows, index_dtype).run(csr=csr)
+    if len(specs) != 1:
+        return SearchPlan(specs, want_rows, index_dtype).run(csr=csr)
+    s = specs[0]
+    key = (_tensor_key(s.qpos), _tensor_key(s.qsup), _tensor_key(s.rpos), _tensor_key(s.rsup),
+           tuple(s.domain_min), tuple(s.domain_max), s.periodic, s.mode, s.hash_map_length, s.search_range,
+           want_rows, index_dtype, s.qpos.device, _stream())
+    plan = _plan_cache.get(key)
+    if plan is None:
+        plan = SearchPlan(specs, want_rows, index_dtype)
+        _plan_cache[key] = plan
+        if len(_plan_cache) > _PLAN_CACHE_MAX:
+            _plan_cache.popitem(last=False)
+    else:
+        _plan_cache.move_to_end(key)
+    return plan.run(csr=csr)
 
 
 def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_levels: int = 4) -> "SearchResult":
