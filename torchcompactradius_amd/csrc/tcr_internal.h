@@ -315,7 +315,9 @@ struct CellIndex {
     const uint64_t *tab;
     uint64_t cap, P;
     bool dense;
+    bool empty;  // no reference points: no cells (the table is not cleared)
     __device__ __forceinline__ int32_t find(int32_t lin) const {
+        if (empty) return -1;
         if (dense) return (uint32_t)lin < P ? ((const int32_t *)tab)[lin] : -1;
         return lintab_find(tab, cap, lin);
     }
@@ -330,6 +332,7 @@ __device__ __forceinline__ CellIndex cell_index(const CloudDev &cd, const uint64
     ci.cap = (uint64_t)cd.lt_cap;
     ci.P = grid_cells(cd);
     ci.dense = cell_index_dense(cd);
+    ci.empty = cd.h.nr == 0;
     return ci;
 }
 

@@ -68,21 +68,21 @@ k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restr
             uint32_t *__restrict__ misc, const int64_t *__restrict__ lt_start, uint64_t *__restrict__ lintab) {
     __shared__ uint32_t h[kMaxPasses][kRadix];
     for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) (&h[0][0])[k] = 0;
-    // empty every cloud's cell index for k_cells (the irregular flag that
-    // picks the dense form is final once k_keys has run): the whole
-    // open-addressing table, or the first (grid cells + 1) / 2 words of a
-    // dense one
-    for (int c = 0; c < B; ++c) {
-        const CloudDev &cd = desc[c];
+    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
+    const CloudDev &cd = desc[ti.c];
+    // the tiles of a cloud empty its cell index for k_cells (the irregular
+    // flag that picks the dense form is final once k_keys has run): the
+    // whole open-addressing table, or the first (grid cells + 1) / 2 words of
+    // a dense one.  A cloud without reference points has no tiles; its
+    // index is never read (CellIndex answers "no cell").
+    {
         const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
-        uint64_t *tab = lintab + lt_start[c];
-        for (int64_t g = (int64_t)blockIdx.x * kSortThreads + threadIdx.x; g < need;
-             g += (int64_t)gridDim.x * kSortThreads)
+        const int64_t ntiles = t_start[ti.c + 1] - ti.first;
+        uint64_t *tab = lintab + lt_start[ti.c];
+        for (int64_t g = (blockIdx.x - ti.first) * kSortThreads + threadIdx.x; g < need; g += ntiles * kSortThreads)
             tab[g] = ~0ull;
     }
     __syncthreads();
-    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
-    const CloudDev &cd = desc[ti.c];
     const bool irr = cloud_wide(cd);
     for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
         const uint32_t ek = eff_key(keys[ti.start + k], irr);
