@@ -6,9 +6,9 @@
 // query position, (2r+1)^d stencil in lexicographic order (dim 0 slowest),
 // periodic neighbour cells through pymod, non-periodic ones unclamped, the
 // bucket lookup matching only the linear index, the exact modDistance and
-// the strict support predicate.  One lane per query; this is the general
-// path that every input shape can take (non-self searches, stencil radius
-// > 1, and the strays of the wavefront-per-cell path).
+// the strict support predicate.  One lane per query (k_query_generic: every
+// query of non-self searches and of stencil radius > 1), or one wavefront per
+// query (k_query_list: the strays of the wavefront-per-cell path).
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -113,6 +113,59 @@ __device__ __forceinline__ int64_t traverse(const Geom &g, const Lookup &lk, con
     return k;
 }
 
+// traverse() with one wavefront per query: the stencil walk is uniform and
+// the lanes test 64 consecutive candidates of a cell at a time; hits are
+// emitted in the same order (emit(k, j) for the k-th hit, from the lanes
+// that hit).  Returns the hit count (uniform).  For queries with many
+// candidates (strays of clouds with few, crowded cells).
+template <int DIM, class Lookup, class Cands, class Emit>
+__device__ __forceinline__ int64_t traverse_wave(const Geom &g, const Lookup &lk, const Cands &cands,
+                                                 const float *xi, float hi, Emit emit) {
+    const int lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int32_t q[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q[d] = cell_coord(xi[d], g.mn[d], g.hcell);
+    const int32_t w = 2 * g.sr + 1;
+    int32_t noff = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) noff *= w;
+    int64_t k = 0;
+    for (int32_t ob = 0; ob < noff; ob += kWave) {
+        // lane l resolves stencil cell ob + l: the lookups run in parallel
+        int64_t lb = 0, le = 0;
+        const int32_t ol = ob + lane;
+        if (ol < noff) {
+            int32_t c[3];
+            int32_t rem = ol;
+#pragma unroll
+            for (int d = DIM - 1; d >= 0; --d) {
+                const int32_t off = (rem % w) - g.sr;
+                rem /= w;
+                c[d] = ((g.pmask >> d) & 1u) ? pymod(q[d] + off, g.nc[d]) : q[d] + off;
+            }
+            if (!lk.find(linear_index<DIM>(c, g.nc), hash_index<DIM>(c, g.L), lb, le)) lb = le = 0;
+        }
+        const int32_t nob = min((int32_t)kWave, noff - ob);
+        for (int32_t t = 0; t < nob; ++t) {  // stencil order
+            const int64_t cb = __shfl(lb, t, kWave), ce = __shfl(le, t, kWave);
+            for (int64_t jb = cb; jb < ce; jb += kWave) {
+                const int64_t j = jb + lane;
+                bool hit = false;
+                if (j < ce) {
+                    float xj[3], hj;
+                    cands.get(j, xj, hj);
+                    hit = support_pred(exact_distance<DIM>(xi, xj, g.mn, g.mx, g.pmask), hi, hj, g.mode);
+                }
+                const uint64_t m = __ballot(hit);
+                if (hit) emit(k + __popcll(m & below), j);
+                k += __popcll(m);
+            }
+        }
+    }
+    return k;
+}
+
 __device__ __forceinline__ Geom cloud_geom(const CloudDev &cd) {
     Geom g;
     for (int d = 0; d < 3; ++d) {
@@ -178,6 +231,38 @@ __device__ __forceinline__ void query_one(const GenericArgs &a, const CloudDev &
     }
 }
 
+// One query per wavefront (every lane holds the same query).
+template <int DIM, bool FILL>
+__device__ __forceinline__ void query_one_wave(const GenericArgs &a, const CloudDev &cd, int c, int64_t g,
+                                               int64_t local) {
+    float xi[3], hi;
+    load_query<DIM>(cd, local, xi, hi);
+    const Geom gm = cloud_geom(cd);
+    const int64_t rs = a.r_start[c];
+    const LinLookup lk{cell_index(cd, a.lintab), a.cell_hash, a.cell_begin, rs};
+    const Float4Cands cands{a.sorted + rs};
+    if (!FILL) {
+        const int64_t n = traverse_wave<DIM>(gm, lk, cands, xi, hi, [](int64_t, int64_t) {});
+        if (__lane_id() == 0) a.counts[g] = (int32_t)n;
+    } else {
+        const int64_t base = a.offsets[g];
+        const int32_t *si = a.sort_index + rs;
+        traverse_wave<DIM>(gm, lk, cands, xi, hi, [&](int64_t k, int64_t j) { a.out.put(base + k, local, si[j]); });
+    }
+}
+
+template <bool FILL>
+__device__ __forceinline__ void generic_query_wave(const GenericArgs &a, int64_t g) {
+    const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
+    const CloudDev &cd = a.desc[c];
+    const int64_t local = g - a.q_start[c];
+    switch (cd.h.dim) {
+        case 1: query_one_wave<1, FILL>(a, cd, c, g, local); break;
+        case 2: query_one_wave<2, FILL>(a, cd, c, g, local); break;
+        default: query_one_wave<3, FILL>(a, cd, c, g, local); break;
+    }
+}
+
 template <bool FILL>
 __device__ __forceinline__ void generic_query(const GenericArgs &a, int64_t g) {
     const int c = find_segment(a.B, g, [&](int k) { return a.q_start[k]; });
@@ -207,13 +292,17 @@ __global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
     generic_query<FILL>(a, g);
 }
 
-// The stray list left by the wavefront-per-cell count pass.
+// The stray list left by the wavefront-per-cell count pass, one wavefront
+// per query (strays come from cells the wavefront-per-cell kernels do not
+// take, e.g. clouds with few, crowded cells, so their candidate lists are
+// long).
 template <bool FILL>
 __global__ void __launch_bounds__(kQBlock) k_query_list(GenericArgs a) {
     if (FILL && *a.total > a.capacity) return;
     const int64_t n = *a.list_count;
-    for (int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kQBlock)
-        generic_query<FILL>(a, a.list[i]);
+    constexpr int kWpb = kQBlock / kWave;
+    for (int64_t i = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave; i < n; i += (int64_t)gridDim.x * kWpb)
+        generic_query_wave<FILL>(a, a.list[i]);
 }
 
 int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
@@ -237,7 +326,7 @@ int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out,
     a.list = w.stray_list;
     a.list_count = w.misc + kMiscStrays;
     if (list) {
-        const dim3 grid(256), block(kQBlock);
+        const dim3 grid(1024), block(kQBlock);
         if (fill) hipLaunchKernelGGL(k_query_list<true>, grid, block, 0, s, a);
         else hipLaunchKernelGGL(k_query_list<false>, grid, block, 0, s, a);
     } else {
