@@ -112,6 +112,19 @@ int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds,
                       int64_t *offsets, void *row, void *col, int32_t index_bytes,
                       int64_t capacity, void *stream);
 
+/* tcr_search_launch for the CSR format: no row indices; the reference's CSR
+ * arrays, indptr int32[nq + 1] and rowEntries int64[nq] (either may be NULL;
+ * py/neighborhood.py:303-305, py/util.py:335-358), are written from the
+ * offsets before the fill.  Same completion (tcr_search_finish) and capacity
+ * rule. */
+int tcr_search_launch_csr(const tcr_cloud *clouds, int32_t n_clouds,
+                          const float *qpos, const float *qsup,
+                          const float *rpos, const float *rsup,
+                          void *workspace, size_t workspace_bytes,
+                          int64_t *offsets, void *col, int32_t index_bytes,
+                          int64_t capacity, int32_t *indptr, int64_t *row_entries,
+                          void *stream);
+
 /* The launches of tcr_search_launch captured once into a HIP graph (on a
  * created stream, not the null stream) and replayed by tcr_graph_launch: the
  * replay reads the same input buffers, workspace and outputs, so a caller

@@ -163,6 +163,35 @@ def test_speculative_capacity_paths():
         np.testing.assert_array_equal(g.col.cpu().numpy(), o.col)
 
 
+@pytest.mark.parametrize("capacity", [0, 10, None])
+def test_csr_arrays_written_by_the_launch(capacity):
+    """The CSR launch (indptr and rowEntries written in the fill phase)
+    against the oracle on a batch of two clouds: with a capacity guess that
+    fits, one that is too small (separate fill) and the shape's hint."""
+    from torchcompactradius_amd.search import SearchPlan
+    rng = np.random.default_rng(77)
+    clouds = [(rng.uniform(-1, 1, (9000, 3)).astype(np.float32), np.full(9000, 0.12, np.float32),
+               [True, True, True], "gather"),
+              (rng.uniform(-1, 1, (700, 2)).astype(np.float32), rng.uniform(0.05, 0.2, 700).astype(np.float32),
+               [False, True], "symmetric")]
+    specs, refs = _specs_and_refs(rng, clouds)
+    plan = SearchPlan(specs, want_rows=False)
+    P = sum(int(o.col.shape[0]) for o in refs)
+    if capacity is None:
+        plan.run()  # leaves the pair count of this shape as the hint
+    cap = None if capacity is None else (P + 4096 if capacity == 0 else capacity)
+    print(f"case: csr launch capacity={cap}", flush=True)
+    res = plan.run(capacity=cap, csr=True)
+    torch.cuda.synchronize()
+    assert res.total == P
+    _check_batch(res, specs, refs)
+    offs = res.offsets.cpu().numpy()
+    assert res.indptr.dtype == torch.int32 and res.row_entries.dtype == torch.int64
+    np.testing.assert_array_equal(res.indptr.cpu().numpy(), offs.astype(np.int32))
+    counts = np.concatenate([o.counts for o in refs]).astype(np.int64)
+    np.testing.assert_array_equal(res.row_entries.cpu().numpy(), counts)
+
+
 def _specs_and_refs(rng, clouds):
     specs, refs = [], []
     for q, h, per, mode in clouds:

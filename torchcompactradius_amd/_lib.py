@@ -57,6 +57,7 @@ _lib: Optional[ctypes.CDLL] = None
 # Every symbol include/tcr.h declares (checked by the ABI tests).
 EXPORTED = (
     "tcr_search_workspace_size", "tcr_search_count", "tcr_search_fill", "tcr_search_launch",
+    "tcr_search_launch_csr",
     "tcr_search_finish", "tcr_search_export", "tcr_timing_enable", "tcr_timing_collect",
     "tcr_graph_capture", "tcr_graph_launch", "tcr_graph_destroy", "tcr_set_tuning", "tcr_get_tuning",
     "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill", "tcr_csr_arrays",
@@ -90,6 +91,8 @@ def load() -> ctypes.CDLL:
     L.tcr_search_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp]
     L.tcr_search_launch.restype = ctypes.c_int
     L.tcr_search_launch.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, i64, vp]
+    L.tcr_search_launch_csr.restype = ctypes.c_int
+    L.tcr_search_launch_csr.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, i32, i64, vp, vp, vp]
     L.tcr_graph_capture.restype = ctypes.c_int
     L.tcr_graph_capture.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, i64,
                                     ctypes.POINTER(ctypes.c_void_p), vp]

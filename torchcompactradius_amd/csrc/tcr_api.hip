@@ -260,7 +260,8 @@ KernelEvents &fill_events() { return g_fill_events; }
 // (skipped on the device if the total exceeds it).  Asynchronous.
 static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup,
                          const float *rpos, const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets,
-                         PairOut out, int64_t capacity, hipStream_t s, Prepared &p, bool allow_timing = true) {
+                         PairOut out, int64_t capacity, hipStream_t s, Prepared &p, bool allow_timing = true,
+                         CsrOut csr = CsrOut{}) {
     int rc = prepare(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, s, p, true);
     if (rc) return rc;
     if (!offsets) {
@@ -292,7 +293,7 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
             ke.before = pe.e[1];
             ke.after = pe.e[3];
         }
-        rc = query_fill(p.b, p.w, offsets, out, capacity, s);
+        rc = query_fill(p.b, p.w, offsets, out, capacity, s, csr);
         if (timed && !ke.recorded) {  // no fill kernel ran
             if (pe.e[1]) hipEventRecord(pe.e[1], s);
             if (pe.e[3]) hipEventRecord(pe.e[3], s);
@@ -387,6 +388,24 @@ int tcr_search_launch(const tcr_cloud *clouds, int32_t n_clouds, const float *qp
     }
     return launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, out,
                          capacity, (hipStream_t)stream, p);
+}
+
+int tcr_search_launch_csr(const tcr_cloud *clouds, int32_t n_clouds, const float *qpos, const float *qsup,
+                          const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes,
+                          int64_t *offsets, void *col, int32_t index_bytes, int64_t capacity, int32_t *indptr,
+                          int64_t *row_entries, void *stream) {
+    Prepared p;
+    PairOut out;
+    if (!pair_out(nullptr, col, index_bytes, out)) return TCR_ERR_ARG;
+    if (capacity < 0) {
+        set_error("capacity must be >= 0");
+        return TCR_ERR_ARG;
+    }
+    CsrOut csr;
+    csr.indptr = indptr;
+    csr.row_entries = row_entries;
+    return launch_search(clouds, n_clouds, qpos, qsup, rpos, rsup, workspace, workspace_bytes, offsets, out,
+                         capacity, (hipStream_t)stream, p, true, csr);
 }
 
 struct tcr_graph_s {

@@ -237,7 +237,13 @@ int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64
 // Count pass (counts[] and the fill's masks); fill pass into offsets,
 // skipped on the device when offsets[nq_total] > capacity.
 int query_count(const Batch &b, Work &w, hipStream_t s);
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s);
+// CSR arrays of a search (either may be null): indptr int32 [nq+1], rowEntries int64 [nq].
+struct CsrOut {
+    int32_t *indptr = nullptr;
+    int64_t *row_entries = nullptr;
+};
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s,
+               CsrOut csr = CsrOut{});
 // Reference-layout bucket tables of every cloud (for export).
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted

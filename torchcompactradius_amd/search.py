@@ -258,10 +258,17 @@ class SearchPlan:
             capacity = None if hint is None else hint + hint // 32 + 256
         if capacity is not None:
             row_buf, col_buf = self._alloc(capacity)
-            check(self.L.tcr_search_launch(*self._args(), ptr(self.offsets), ptr(row_buf), ptr(col_buf),
-                                           self.index_bytes, capacity, stream))
-            if csr:
-                ip, re = self._csr(stream)
+            if csr and row_buf is None:
+                # the CSR arrays are written in the fill phase, before the host waits
+                ip = torch.empty(self.nq + 1, dtype=torch.int32, device=self.device)
+                re = torch.empty(self.nq, dtype=torch.int64, device=self.device)
+                check(self.L.tcr_search_launch_csr(*self._args(), ptr(self.offsets), ptr(col_buf), self.index_bytes,
+                                                   capacity, ptr(ip), ptr(re), stream))
+            else:
+                check(self.L.tcr_search_launch(*self._args(), ptr(self.offsets), ptr(row_buf), ptr(col_buf),
+                                               self.index_bytes, capacity, stream))
+                if csr:
+                    ip, re = self._csr(stream)
             P = self._finish(stream)
             _remember(self.key, P)
             if P <= capacity:
