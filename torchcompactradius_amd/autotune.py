@@ -68,11 +68,19 @@ def autotune(plan, reps: int = 3, cell_candidates: Iterable[int] = CELL_CANDIDAT
     return plan.tuning
 
 
+_installed: list = ["unset"]  # the launch shape last set through install()
+
+
 def install(tuning: Optional[Tuple[int, int]]) -> None:
-    """Set the library's launch shape (None: automatic)."""
+    """Set the library's launch shape (None: automatic); a repeat of the
+    shape last set is a no-op."""
+    key = None if tuning is None else (int(tuning[0]), int(tuning[1]))
+    if _installed[0] == key:
+        return
     L = _lib.load()
-    if tuning is None:
+    if key is None:
         _lib.check(L.tcr_set_tuning(None))
     else:
-        t = TcrTuning(int(tuning[0]), int(tuning[1]))
+        t = TcrTuning(*key)
         _lib.check(L.tcr_set_tuning(ctypes.byref(t)))
+    _installed[0] = key

@@ -79,7 +79,10 @@ def periodic_mask(periodic, dim: int) -> int:
 
 def _host_floats(v, dim: int, name: str) -> List[float]:
     if isinstance(v, torch.Tensor):
-        v = v.detach().to("cpu", torch.float32).reshape(-1).tolist()
+        if v.device.type == "cpu" and v.dtype == torch.float32 and v.dim() == 1:
+            v = v.tolist()
+        else:
+            v = v.detach().to("cpu", torch.float32).reshape(-1).tolist()
     v = [float(x) for x in v]
     if len(v) != dim:
         raise ValueError(f"{name} has {len(v)} entries, expected {dim}")
