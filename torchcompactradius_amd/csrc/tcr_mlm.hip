@@ -346,10 +346,14 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
     if (!FILL) a.counts[g] = (int32_t)k;
 }
 
+// One thread per query.  A self-search takes its queries in sorted order
+// (thread t: the point at sorted slot t), so that the lanes of a wavefront
+// hold neighbouring points of one level and walk the same cells.
 template <bool FILL>
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_query(MlmArgs a) {
-    const int64_t g = (int64_t)blockIdx.x * kMlmBlock + threadIdx.x;
-    if (g >= a.nq) return;
+    const int64_t t = (int64_t)blockIdx.x * kMlmBlock + threadIdx.x;
+    if (t >= a.nq) return;
+    const int64_t g = a.desc[0].h.self_search ? (int64_t)a.sort_index[t] : t;
     switch (a.desc[0].h.dim) {
         case 1: mlm_query<1, FILL>(a, g); break;
         case 2: mlm_query<2, FILL>(a, g); break;
