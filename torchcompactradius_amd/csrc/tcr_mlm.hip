@@ -392,11 +392,42 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One wavefront per row: sort the row's (offset << 32 | index) keys and
-// write the original indices.  Rows up to kMlmSortMax sort in LDS (bitonic
-// network over the next power of two, padded with all-ones keys); lane 0
-// sorts longer rows in place (insertion sort) and writes them.  `scratch`
-// may alias the int64 column output: every entry is read before it is
-// overwritten.
+// write the original indices.  Two launches: SHORT takes the rows of at
+// most 64 keys and sorts them in registers (one key per lane, bitonic
+// network over shuffles; no LDS, so full occupancy); the other takes the
+// longer rows, up to kMlmSortMax in LDS (bitonic network over the next power
+// of two, padded with all-ones keys), and has lane 0 sort still longer rows
+// in place (insertion sort).  `scratch` may alias the int64 column output:
+// every entry is read before it is overwritten.
+__device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t m, const uint64_t *scratch,
+                                               const PairOut &out) {
+    const int lane = __lane_id();
+    uint64_t key = lane < m ? scratch[b + lane] : ~0ull;
+#pragma unroll
+    for (int kk = 2; kk <= kWave; kk <<= 1)
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
+            const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), j, kWave);
+            const uint64_t other = ((uint64_t)ohi << 32) | olo;
+            // pair (i, i ^ j): the lower lane keeps the minimum in an
+            // ascending block (lane & kk == 0), the maximum otherwise
+            const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
+            key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+        }
+    if (lane < m) out.put(b + lane, g, (int32_t)(uint32_t)key);
+}
+
+__global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_short(int64_t nq, const int64_t *__restrict__ offsets,
+                                                              const uint64_t *scratch, PairOut out) {
+    const int64_t wid = ((int64_t)blockIdx.x * kMlmBlock + threadIdx.x) / kWave;
+    const int64_t nw = ((int64_t)gridDim.x * kMlmBlock) / kWave;
+    for (int64_t g = wid; g < nq; g += nw) {
+        const int64_t b = offsets[g], m = offsets[g + 1] - b;
+        if (m > 0 && m <= kWave) sort_row_short(g, b, m, scratch, out);
+    }
+}
+
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const int64_t *__restrict__ offsets,
                                                              uint64_t *scratch, PairOut out) {
     __shared__ uint64_t s_keys[kMlmBlock / kWave][kMlmSortMax];
@@ -406,7 +437,7 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const i
     const int64_t nw = ((int64_t)gridDim.x * kMlmBlock) / kWave;
     for (int64_t g = wid; g < nq; g += nw) {
         const int64_t b = offsets[g], m = offsets[g + 1] - b;
-        if (m <= 0) continue;
+        if (m <= kWave) continue;  // k_mlm_sort_short
         if (m <= kMlmSortMax) {
             int n2 = 1;
             while (n2 < m) n2 <<= 1;
@@ -446,6 +477,8 @@ int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scrat
     if (b.nq_total == 0) return TCR_OK;
     int64_t blocks = (b.nq_total + 3) / 4;
     if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_mlm_sort_short, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets,
+                       (const uint64_t *)scratch, out);
     hipLaunchKernelGGL(k_mlm_sort_rows, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets, scratch,
                        out);
     TCR_LAUNCH_CHECK();
