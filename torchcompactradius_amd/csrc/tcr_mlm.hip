@@ -38,6 +38,8 @@
 // points outside the hCell grid, fewer than 3 cells in a dimension (the
 // compact stencil would then revisit or alias cells), level grids beyond
 // 2^29 cells.
+#include <algorithm>
+
 #include "tcr_internal.h"
 
 namespace tcr {
@@ -127,50 +129,72 @@ __device__ __forceinline__ void mlm_key_one(const CloudDev &cd, int64_t g, int L
     level = l;
 }
 
+// Grid-stride over the points; each thread keeps its running maximum
+// support per level and its flags, merged per block before one atomic per
+// level (a per-wavefront atomic on the same four words serialised).
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_keys(CloudDev *__restrict__ desc, int64_t n, int L,
                                                         float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
                                                         uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags,
                                                         uint32_t *__restrict__ misc) {
-    const int64_t g = (int64_t)blockIdx.x * kMlmBlock + threadIdx.x;
+    constexpr int kMaxLevels = 4;
+    __shared__ uint32_t s_max[kMlmBlock / kWave][kMaxLevels + 1];
     const CloudDev &cd = desc[0];
     const float hmax = ordered_to_float(cd.hmax_ord);
-    int level = -1;
-    float h = 0.0f;
+    uint32_t lmax[kMaxLevels] = {0u, 0u, 0u, 0u};
     uint32_t flags = 0;
-    if (g < n) {
-        uint32_t key;
+    for (int64_t g = (int64_t)blockIdx.x * kMlmBlock + threadIdx.x; g < n; g += (int64_t)gridDim.x * kMlmBlock) {
+        uint32_t key, f = 0;
         float4 wp;
+        int level = -1;
         switch (cd.h.dim) {
-            case 1: mlm_key_one<1>(cd, g, L, hmax, key, wp, level, flags); break;
-            case 2: mlm_key_one<2>(cd, g, L, hmax, key, wp, level, flags); break;
-            default: mlm_key_one<3>(cd, g, L, hmax, key, wp, level, flags); break;
+            case 1: mlm_key_one<1>(cd, g, L, hmax, key, wp, level, f); break;
+            case 2: mlm_key_one<2>(cd, g, L, hmax, key, wp, level, f); break;
+            default: mlm_key_one<3>(cd, g, L, hmax, key, wp, level, f); break;
         }
         rwrap[g] = wp;
         keys[g] = key;
         vals[g] = (uint32_t)g;
         pflags[g] = 0;
-        h = wp.w;
-    }
-    // per-level maximum support: wavefront reductions, one atomic per level
-    for (int l = 0; l < L; ++l) {
-        uint32_t m = (level == l && h >= 0.0f) ? float_to_ordered(h) : 0u;
+        flags |= f;
+        if (wp.w >= 0.0f) {
+            const uint32_t o = float_to_ordered(wp.w);
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
-        if (m && (threadIdx.x & (kWave - 1)) == 0) atomicMax(&misc[kMiscLevelMax + l], m);
+            for (int l = 0; l < kMaxLevels; ++l)
+                if (level == l) lmax[l] = max(lmax[l], o);
+        }
     }
-    uint32_t f = flags;
+    // per-level maximum support and flags: wavefront, block, then one atomic each
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
 #pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) f |= __shfl_xor(f, o, kWave);
-    if (f && (threadIdx.x & (kWave - 1)) == 0) atomicOr(&misc[kMiscMlmFlags], f);
+    for (int l = 0; l < kMaxLevels; ++l) {
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) lmax[l] = max(lmax[l], __shfl_xor(lmax[l], o, kWave));
+        if (lane == 0) s_max[wave][l] = lmax[l];
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) flags |= __shfl_xor(flags, o, kWave);
+    if (lane == 0) s_max[wave][kMaxLevels] = flags;
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevels) {
+        uint32_t m = 0;
+        for (int w = 0; w < kMlmBlock / kWave; ++w) m = threadIdx.x < kMaxLevels ? max(m, s_max[w][threadIdx.x])
+                                                                                 : (m | s_max[w][kMaxLevels]);
+        if (threadIdx.x < kMaxLevels) {
+            if (m && (int)threadIdx.x < L) atomicMax(&misc[kMiscLevelMax + threadIdx.x], m);
+        } else if (m) {
+            atomicOr(&misc[kMiscMlmFlags], m);
+        }
+    }
     // the level keys use 31 bits: every radix pass runs
-    if (g == 0) desc[0].irregular = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) desc[0].irregular = 1;
 }
 
 int mlm_keys_launch(const Batch &b, Work &w, int levels, hipStream_t s) {
     const int64_t n = b.nr_total;
     if (n == 0) return TCR_OK;
-    hipLaunchKernelGGL(k_mlm_keys, dim3((unsigned)((n + kMlmBlock - 1) / kMlmBlock)), dim3(kMlmBlock), 0, s,
-                       w.desc, n, levels, w.rwrap, w.keys0, w.vals0, w.pflags, w.misc);
+    const int64_t blocks = std::min<int64_t>(1024, (n + kMlmBlock - 1) / kMlmBlock);
+    hipLaunchKernelGGL(k_mlm_keys, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, w.desc, n, levels, w.rwrap, w.keys0,
+                       w.vals0, w.pflags, w.misc);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
