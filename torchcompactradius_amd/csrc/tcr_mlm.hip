@@ -427,8 +427,11 @@ __device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t m, 
                                                const PairOut &out) {
     const int lane = __lane_id();
     uint64_t key = lane < m ? scratch[b + lane] : ~0ull;
+    // a network over the next power of two >= m: lanes beyond it never pair
+    // with lanes below it
 #pragma unroll
-    for (int kk = 2; kk <= kWave; kk <<= 1)
+    for (int kk = 2; kk <= kWave; kk <<= 1) {
+        if ((kk >> 1) >= m) break;  // uniform: the first kk/2 lanes hold the whole row
 #pragma unroll
         for (int j = kk >> 1; j > 0; j >>= 1) {
             const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
@@ -439,6 +442,7 @@ __device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t m, 
             const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
             key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
         }
+    }
     if (lane < m) out.put(b + lane, g, (int32_t)(uint32_t)key);
 }
 
