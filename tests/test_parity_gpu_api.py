@@ -7,7 +7,8 @@ import pytest
 import torch
 
 import oracle
-from torchcompactradius_amd import DomainDescription, PointCloud, native_ops, radius, radiusSearch, synthetic
+from torchcompactradius_amd import (DomainDescription, PointCloud, coo_to_csr, native_ops, neighborSearchExisting,
+                                    radius, radiusSearch, synthetic)
 from torchcompactradius_amd.search import CloudSpec, run_search
 
 pytestmark = pytest.mark.gpu
@@ -117,6 +118,71 @@ def test_native_ops_on_exported_structure(mode):
     np.testing.assert_array_equal(counts.cpu().numpy(), o.counts)
     np.testing.assert_array_equal(i.cpu().numpy(), o.row)
     np.testing.assert_array_equal(j.cpu().numpy(), o.col)
+
+
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("self_search", [False, True])
+def test_search_existing_matches_radius_search(mode, self_search):
+    """neighborSearchExisting (neighborhood.py:28-132) on the structure of a
+    radiusSearch gives that search's result for its (once-wrapped) queries,
+    in COO and CSR, and the oracle's pairs."""
+    w = synthetic.periodic_3d(n=18)
+    rng = np.random.default_rng(31)
+    sup = (w.sup * rng.uniform(0.7, 1.3, w.sup.shape[0])).astype(np.float32)
+    dom = DomainDescription(torch.tensor(w.domain_min), torch.tensor(w.domain_max), torch.tensor(w.periodic), 3)
+    if self_search:
+        qp, qs, ref = w.pos, sup, None
+    else:
+        qp = rng.uniform(-1.2, 1.2, (3000, 3)).astype(np.float32)   # some outside the domain: wrapped
+        qs = (w.sup[0] * rng.uniform(0.6, 1.4, 3000)).astype(np.float32)
+        ref = PointCloud(_t(w.pos), _t(sup))
+    print(f"case: search existing mode={mode} self={self_search}", flush=True)
+    out, hm = radiusSearch(PointCloud(_t(qp), _t(qs)), ref, mode=mode, domain=dom, hashMapLength=1009,
+                           algorithm="compact", format="coo", returnStructure=True)
+    qw = PointCloud(_t(oracle.wrap(qp, w.domain_min, w.domain_max, w.periodic)), _t(qs))
+    got = neighborSearchExisting(qw, hm, mode=mode)
+    torch.testing.assert_close(got.row, out.row, rtol=0, atol=0)
+    torch.testing.assert_close(got.col, out.col, rtol=0, atol=0)
+    assert (got.numRows, got.numCols) == (out.numRows, out.numCols)
+    csr = neighborSearchExisting(qw, hm, mode=mode, format="csr")
+    want = coo_to_csr(out, isSorted=True)
+    for a, b in ((csr.indices, want.indices), (csr.indptr, want.indptr), (csr.rowEntries, want.rowEntries)):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+    o = oracle.radius_search(qp, qs, None if self_search else w.pos, None if self_search else sup,
+                             domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic, mode=mode,
+                             hash_map_length=1009)
+    np.testing.assert_array_equal(got.row.cpu().numpy(), o.row)
+    np.testing.assert_array_equal(got.col.cpu().numpy(), o.col)
+
+
+def test_search_existing_fixed_support():
+    """A float support (on the query cloud, or the structure's fixedSupport)
+    acts as that support on every point: the structure of a supportOverride
+    search then reproduces that search."""
+    w = synthetic.periodic_3d(n=16)
+    rng = np.random.default_rng(32)
+    qp = rng.uniform(-1.0, 1.0, (2000, 3)).astype(np.float32)
+    h = float(w.sup[0])
+    dom = DomainDescription(torch.tensor(w.domain_min), torch.tensor(w.domain_max), torch.tensor(w.periodic), 3)
+    print("case: search existing fixed support", flush=True)
+    out, hm = radiusSearch(PointCloud(_t(qp), None), PointCloud(_t(w.pos), None), supportOverride=h,
+                           mode="symmetric", domain=dom, hashMapLength=4096, algorithm="compact",
+                           returnStructure=True)
+    qw = _t(oracle.wrap(qp, w.domain_min, w.domain_max, w.periodic))
+    for got in (neighborSearchExisting(PointCloud(qw, h), hm),
+                neighborSearchExisting(PointCloud(qw, None), hm._replace(fixedSupport=h))):
+        torch.testing.assert_close(got.row, out.row, rtol=0, atol=0)
+        torch.testing.assert_close(got.col, out.col, rtol=0, atol=0)
+
+
+def test_search_existing_empty_query():
+    w = synthetic.periodic_3d(n=10)
+    dom = DomainDescription(torch.tensor(w.domain_min), torch.tensor(w.domain_max), torch.tensor(w.periodic), 3)
+    _, hm = radiusSearch(PointCloud(_t(w.pos), _t(w.sup)), None, domain=dom, algorithm="compact",
+                         returnStructure=True)
+    got = neighborSearchExisting(PointCloud(torch.zeros((0, 3), device=DEV), torch.zeros(0, device=DEV)), hm,
+                                 mode="gather")
+    assert got.row.numel() == 0 and got.col.numel() == 0 and got.numRows == 0
 
 
 def test_native_ops_errors():

@@ -22,3 +22,18 @@ def test_compute_h_workload_values():
     assert np.float32(compute_h([-1, -1], [1, 1], np.float32(np.sqrt(32 * (2 / 32) ** 2 / np.pi)))) == np.float32(0.2)
     h = np.float32((3 * 50 * 0.02 ** 3 / (4 * np.pi)) ** (1 / 3))
     assert abs(compute_h([-1] * 3, [1] * 3, h) - 0.046512) < 1e-6
+
+
+def test_search_existing_variant_errors():
+    """neighborSearchExisting's variant dispatch (neighborhood.py:78-108):
+    unknown variants raise ValueError before any device work; the per-point
+    Python variants are not provided."""
+    import pytest
+    import torch
+    from torchcompactradius_amd import PointCloud, neighborSearchExisting
+    q = PointCloud(torch.zeros((4, 3)), torch.ones(4))
+    with pytest.raises(ValueError):
+        neighborSearchExisting(q, None, variant="fortran")
+    for v in ("python", "pythonDynamic"):
+        with pytest.raises(NotImplementedError):
+            neighborSearchExisting(q, None, variant=v)

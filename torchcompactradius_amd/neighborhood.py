@@ -1,11 +1,13 @@
-"""Public search API: ``radiusSearch`` and ``radius``.
+"""Public search API: ``radiusSearch``, ``neighborSearchExisting`` and ``radius``.
 
 Mirrors torchCompactRadius 0.5.5 ``src/torchCompactRadius/neighborhood.py``:
 ``radiusSearch`` (:136-309) with the same arguments, assertions and result
-types; ``algorithm='compact'`` (:283) runs on libtcr_hip.so.  ``radius``
-(:313-365) implements the torch_cluster-compatible behaviour the reference
-documents (README.md:45) -- flipped ``(i, j)`` order, per-batch search with
-index offsets -- on top of the batched search.
+types; ``algorithm='compact'`` (:283) runs on libtcr_hip.so.
+``neighborSearchExisting`` (:28-132) searches query points against an
+exported structure.  ``radius`` (:313-365) implements the
+torch_cluster-compatible behaviour the reference documents (README.md:45) --
+flipped ``(i, j)`` order, per-batch search with index offsets -- on top of
+the batched search.
 """
 from __future__ import annotations
 
@@ -15,7 +17,7 @@ import torch
 
 from ._lib import Unsupported
 from .search import CloudSpec, CompactHashMap, export_structure, periodic_mask, run_mlm, run_search, _host_floats
-from .util import DomainDescription, PointCloud, SparseCOO, SparseCSR
+from .util import DomainDescription, PointCloud, SparseCOO, SparseCSR, coo_to_csr
 
 ALGORITHMS = ("naive", "small", "compact", "mlm")
 
@@ -122,6 +124,74 @@ def radiusSearch(
         if returnStructure:
             return out, export_structure(res, 0, spec)
         return out
+
+
+def neighborSearchExisting(
+        queryPointCloud: PointCloud,
+        hashMap: CompactHashMap, mode: str = "symmetric", searchRadius: int = 1, variant: str = "cpp",
+        verbose: bool = False, format: str = "coo") -> Union[SparseCOO, SparseCSR]:
+    """Search query points against an existing structure (neighborhood.py:28-132),
+    e.g. the ``CompactHashMap`` of ``radiusSearch(..., returnStructure=True)``.
+
+    Runs searchNeighbors_cpp's sequence (compactHashing/cppSearch.py:60-98)
+    on the structure's tensors through the reference-layout ops of
+    libtcr_hip.so: countNeighbors, int32 exclusive offsets,
+    buildNeighborList, ``j = sortIndex[j]``; rows are query indices, columns
+    reference indices.  Query positions are used as given: ``radiusSearch``
+    wraps its queries into a periodic domain once before searching, so wrapped
+    positions reproduce its result.
+
+    The reference function reads ``hashMap.periodic`` (neighborhood.py:81), a
+    field its record does not have (compactHashing/datastructure.py:26 names it
+    ``periodicity``), so there every call raises; this is the search that code
+    describes.  A float support -- ``queryPointCloud.supports`` given as a
+    float, or ``hashMap.fixedSupport`` when the query has no supports -- is
+    applied as that support to every query and reference point (the
+    reference's fixed-support ops are commented out of its build).  The
+    per-point Python variants ('python', 'pythonDynamic') are not provided.
+    """
+    with torch.no_grad():
+        if variant in ("python", "pythonDynamic"):
+            raise NotImplementedError(f"variant='{variant}' is not provided by torchcompactradius_amd; use 'cpp'")
+        if variant != "cpp":
+            raise ValueError("variant must be either cpp, python or pythonDynamic")
+        assert format in ["coo", "csr"], f"format = {format} not supported"
+        from . import native_ops
+        qpos = queryPointCloud.positions
+        sup = queryPointCloud.supports
+        fixed: Optional[float] = None
+        if sup is None:
+            qsup = None
+            fixed = hashMap.fixedSupport
+        elif isinstance(sup, torch.Tensor):
+            qsup = sup
+        else:
+            qsup = None
+            fixed = float(sup)
+        sorted_pos = hashMap.sortedPositions
+        sorted_sup = hashMap.sortedSupports
+        nq, nr = int(qpos.shape[0]), int(sorted_pos.shape[0])
+        if fixed is not None:
+            qsup = torch.full((nq,), fixed, dtype=torch.float32, device=qpos.device)
+            sorted_sup = torch.full((nr,), fixed, dtype=torch.float32, device=sorted_pos.device)
+        hcell = float(hashMap.hCell.item()) if isinstance(hashMap.hCell, torch.Tensor) else float(hashMap.hCell)
+        args = (qpos, qsup, searchRadius, sorted_pos, sorted_sup, hashMap.hashTable, hashMap.hashMapLength,
+                hashMap.numCells, hashMap.sortedCellTable, hashMap.qMin, hcell, hashMap.maxD, hashMap.minD,
+                hashMap.periodicity, mode, verbose)
+        if verbose:
+            print(f"neighborSearchExisting: nq={nq} nr={nr} mode={mode} searchRadius={searchRadius} "
+                  f"fixedSupport={fixed} format={format}")
+        counts = native_ops.countNeighbors(*args)
+        offsets = torch.zeros(nq, dtype=torch.int32, device=qpos.device)
+        if nq > 1:
+            torch.cumsum(counts[:-1], 0, dtype=torch.int32, out=offsets[1:])
+        total = int(offsets[-1].item() + counts[-1].item()) if nq else 0
+        i, j = native_ops.buildNeighborList(counts, offsets, total, *args)
+        j = hashMap.sortIndex[j]
+        sparse = SparseCOO(i, j, nq, nr)
+        if format == "csr":
+            return coo_to_csr(sparse, isSorted=True)
+        return sparse
 
 
 def radius(queryPositions: torch.Tensor,
