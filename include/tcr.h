@@ -230,6 +230,34 @@ int tcr_build_neighbor_list(const int32_t *counts, const int32_t *offsets, int64
  * py/neighborhood.py:303-305 / py/util.py:335-358. */
 int tcr_csr_arrays(const int64_t *offsets, int64_t nq, int32_t *indptr, int64_t *row_entries, void *stream);
 
+/* Brute-force search with the semantics of radiusSearch's default
+ * algorithm='naive': radiusNaive / radiusNaiveFixed
+ * (py/searchAlgorithms/radiusNaive.py:8-56, called at
+ * py/neighborhood.py:226-229,268-271).  Every (query a, reference b) pair with
+ * distance <= the mode's support (radiusNaive's own periodic `mod`, fp32
+ * squares summed in dimension order, sqrtf; superSymmetric is
+ * max(hx[a], hy[a]) and needs nx == ny), ordered by reference index, then
+ * query index.  x fp32[nx, dim], y fp32[ny, dim], hx/hy fp32 (device; NULL
+ * where the mode reads none); domain_min/max fp32[dim] on the HOST; bit d of
+ * periodic_mask = dim d periodic; periodic coordinates are wrapped `wraps`
+ * (0..2) times first (radiusSearch wraps once, radiusNaive once more).
+ * fixed_support non-NULL selects radiusNaiveFixed: squared distance <=
+ * (*fixed_support)^2, supports unused.
+ * tcr_naive_count writes offsets int64[ny + 1] (first slot per reference
+ * point, offsets[ny] = pair count); tcr_naive_fill, with the same inputs and
+ * those offsets, writes ii (query) and jj (reference) indices, index_bytes 4
+ * or 8 each.  Workspace: tcr_naive_workspace_size bytes (device). */
+size_t tcr_naive_workspace_size(int64_t nx, int64_t ny, int32_t dim);
+int tcr_naive_count(const float *x, const float *hx, int64_t nx, const float *y, const float *hy, int64_t ny,
+                    int32_t dim, const float *domain_min, const float *domain_max, uint32_t periodic_mask,
+                    int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
+                    size_t workspace_bytes, int64_t *offsets, void *stream);
+int tcr_naive_fill(const float *x, const float *hx, int64_t nx, const float *y, const float *hy, int64_t ny,
+                   int32_t dim, const float *domain_min, const float *domain_max, uint32_t periodic_mask,
+                   int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
+                   size_t workspace_bytes, const int64_t *offsets, void *ii, void *jj, int32_t index_bytes,
+                   void *stream);
+
 /* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
  * (no counterpart in the reference; a benchmark hook).  While enabled, each
  * launch records HIP events on its stream that delimit the count phase

@@ -2,7 +2,8 @@
 
 ``oracle/tcr_oracle.c`` restates torchCompactRadius 0.5.5's
 ``radiusSearch(..., algorithm='compact')`` in plain C + OpenMP; this module is
-its ctypes binding plus numpy helpers.  Only ``tests/``,
+its ctypes binding plus numpy helpers.  ``oracle/naive.py`` restates the
+brute-force ``algorithm='naive'`` search in numpy.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import it,
 as the checker; the product package never does.
 """
@@ -15,6 +16,8 @@ from dataclasses import dataclass
 from typing import Optional
 
 import numpy as np
+
+from .naive import naive_mod, radius_naive, radius_naive_fixed, radius_search_naive, wrap_once  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtcr_oracle.so")

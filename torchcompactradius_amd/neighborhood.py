@@ -54,8 +54,10 @@ def radiusSearch(
     indices, columns reference indices, row-sorted; within a row the order is
     the reference compact path's (cell stencil order, then original index).
 
-    ``algorithm='compact'`` and ``'mlm'`` are computed by this package, every
-    search on the GPU.  ``'mlm'`` -- the multi-level search for variable
+    ``algorithm='compact'``, ``'mlm'`` and ``'naive'`` are computed by this
+    package, every search on the GPU; ``'naive'`` keeps the reference's
+    brute-force semantics (``<=``, its own periodic ``mod``, pairs ordered by
+    reference index).  ``'mlm'`` -- the multi-level search for variable
     supports -- returns exactly the compact result (pairs and order); for
     inputs it does not take (gather mode, points outside the grid, ...) and
     with ``returnStructure`` it runs the compact search.  ``supportOverride`` maps a scalar support onto every
@@ -83,9 +85,12 @@ def radiusSearch(
         if referencePointCloud is not None and referencePointCloud.supports is not None:
             assert referencePointCloud.supports.shape[0] == referencePointCloud.positions.shape[0], \
                 "support.shape[0] != referencePositions.shape[0]"
+        if algorithm == "naive":
+            return _naive(queryPointCloud, referencePointCloud, supportOverride, mode, domain, verbose, format,
+                          returnStructure, indexDtype)
         if algorithm not in ("compact", "mlm"):
             raise NotImplementedError(
-                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use 'compact' or 'mlm'")
+                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use 'compact', 'mlm' or 'naive'")
 
         qpos = queryPointCloud.positions
         qsup = queryPointCloud.supports
@@ -124,6 +129,41 @@ def radiusSearch(
         if returnStructure:
             return out, export_structure(res, 0, spec)
         return out
+
+
+def _naive(queryPointCloud: PointCloud, referencePointCloud: Optional[PointCloud], supportOverride: Optional[float],
+           mode: str, domain: Optional[DomainDescription], verbose: bool, format: str, returnStructure: bool,
+           indexDtype: torch.dtype):
+    """radiusSearch's algorithm='naive' branch (neighborhood.py:186-271,
+    298-309): with a domain, periodic coordinates are wrapped once here
+    (getPeriodicPointCloud) and once more by radiusNaive; without one the
+    default domain [0,1]^d is not periodic.  The pairs come in radiusNaive's
+    order -- by reference index, then query index -- and the CSR form is
+    coo_to_csr(isSorted=True) of them, as the reference builds it (its indices
+    are therefore in that order too).  supportOverride selects radiusNaiveFixed
+    (squared distance <= support^2), the branch the reference dispatches to
+    (:226-229) but cannot run (it passes PointCloud records where the scripted
+    function takes tensors)."""
+    from .search import _f32, run_naive
+    qpos = _f32(queryPointCloud.positions, "queryPointCloud.positions")
+    qsup = _f32(queryPointCloud.supports, "queryPointCloud.supports")
+    if referencePointCloud is None:
+        rpos, rsup = qpos, qsup
+    else:
+        rpos = _f32(referencePointCloud.positions, "referencePointCloud.positions")
+        rsup = _f32(referencePointCloud.supports, "referencePointCloud.supports")
+    dim = qpos.shape[1]
+    mn, mx, pmask = _domain(domain, dim)
+    wraps = 2 if domain is not None else 0
+    if verbose:
+        print("Calling radiusNaiveFixed:" if supportOverride is not None else "Calling radiusNaive")
+    ii, jj = run_naive(qpos, qsup, rpos, rsup, mn, mx, pmask, wraps, mode, supportOverride, indexDtype)
+    out: Union[SparseCOO, SparseCSR] = SparseCOO(ii, jj, qpos.shape[0], rpos.shape[0])
+    if format == "csr":
+        out = coo_to_csr(out, isSorted=True)
+    if returnStructure:
+        return out, None
+    return out
 
 
 def neighborSearchExisting(

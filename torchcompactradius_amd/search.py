@@ -395,6 +395,42 @@ def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_level
     return SearchResult(offsets, row, col, P, ws, clouds)
 
 
+def run_naive(qpos: torch.Tensor, qsup: Optional[torch.Tensor], rpos: torch.Tensor, rsup: Optional[torch.Tensor],
+              domain_min: Sequence[float], domain_max: Sequence[float], periodic: int, wraps: int, mode: str,
+              fixed_support: Optional[float] = None, index_dtype=torch.int64):
+    """Brute-force search with the semantics of algorithm='naive'
+    (searchAlgorithms/radiusNaive.py:8-56) on libtcr_hip.so: (ii, jj) query
+    and reference indices ordered by reference index, then query index."""
+    if index_dtype not in (torch.int64, torch.int32):
+        raise ValueError("index_dtype must be torch.int64 or torch.int32")
+    L = _lib.load()
+    for t, name in ((qpos, "queryPointCloud.positions"), (rpos, "referencePointCloud.positions"),
+                    (qsup, "queryPointCloud.supports"), (rsup, "referencePointCloud.supports")):
+        if t is not None:
+            _require_device(t, name)
+    device = qpos.device
+    nx, ny, dim = int(qpos.shape[0]), int(rpos.shape[0]), int(qpos.shape[1])
+    mn = (ctypes.c_float * 3)(*[float(v) for v in domain_min[:dim]])
+    mx = (ctypes.c_float * 3)(*[float(v) for v in domain_max[:dim]])
+    fixed = ctypes.byref(ctypes.c_double(float(fixed_support))) if fixed_support is not None else None
+    ws_bytes = L.tcr_naive_workspace_size(nx, ny, dim)
+    if ws_bytes == 0:
+        check(_lib.TCR_ERR_ARG)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    offsets = torch.empty(ny + 1, dtype=torch.int64, device=device)
+    stream = _stream()
+    args = (ptr(qpos), ptr(qsup), nx, ptr(rpos), ptr(rsup), ny, dim, mn, mx, int(periodic), int(wraps),
+            MODES[mode], fixed, ptr(ws), ws_bytes)
+    check(L.tcr_naive_count(*args, ptr(offsets), stream))
+    total = int(offsets[-1].item())
+    ii = torch.empty(total, dtype=index_dtype, device=device)
+    jj = torch.empty(total, dtype=index_dtype, device=device)
+    if total > 0:
+        check(L.tcr_naive_fill(*args, ptr(offsets), ptr(ii), ptr(jj), 8 if index_dtype == torch.int64 else 4,
+                               stream))
+    return ii, jj
+
+
 def export_structure(res: SearchResult, cloud: int, spec: CloudSpec) -> CompactHashMap:
     """Copy one cloud's structure out of the workspace (returnStructure)."""
     L = _lib.load()
