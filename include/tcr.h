@@ -20,6 +20,12 @@
  *   tcr_search_export          <- the CompactHashMap record returned with
  *                                 returnStructure=True
  *                                 (py/searchAlgorithms/compactSearch.py:125-143)
+ *   tcr_naive_count/_fill      <- radiusNaive / radiusNaiveFixed
+ *                                 (py/searchAlgorithms/radiusNaive.py:8-56),
+ *                                 radiusSearch's default algorithm='naive'
+ *   tcr_small_count/_fill      <- neighborSearchSmall / neighborSearchSmallFixed
+ *                                 (cpp/neighborhoodSmall_cpu.cpp:45-302, commented
+ *                                 out of the reference build), algorithm='small'
  *
  * Errors: every entry point returns TCR_OK (0) or a negative code; the
  * message of the last failure on the calling thread is tcr_last_error().
@@ -253,6 +259,25 @@ int tcr_naive_count(const float *x, const float *hx, int64_t nx, const float *y,
                     int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
                     size_t workspace_bytes, int64_t *offsets, void *stream);
 int tcr_naive_fill(const float *x, const float *hx, int64_t nx, const float *y, const float *hy, int64_t ny,
+                   int32_t dim, const float *domain_min, const float *domain_max, uint32_t periodic_mask,
+                   int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
+                   size_t workspace_bytes, const int64_t *offsets, void *ii, void *jj, int32_t index_bytes,
+                   void *stream);
+
+/* Brute-force search with the semantics of algorithm='small', the op the
+ * reference keeps commented out (cpp/neighborhoodSmall_cpu.cpp:45-302; its
+ * Python binding is None at py/compactHashing/cppWrapper.py:87-88, so the
+ * reference raises): for each query i, every reference j in index order with
+ * modDistance (cpp/common.h:106-129) < the mode's support (superSymmetric
+ * std::max(h_i, h_j)); fixed_support non-NULL: modDistance2 < h*h with h*h in
+ * fp32.  Same arguments and workspace as tcr_naive_*, except that offsets is
+ * int64[nx + 1] (first slot per query) and the pairs come in (query,
+ * reference) order; radiusSearch passes positions wrapped once (wraps = 1). */
+int tcr_small_count(const float *x, const float *hx, int64_t nx, const float *y, const float *hy, int64_t ny,
+                    int32_t dim, const float *domain_min, const float *domain_max, uint32_t periodic_mask,
+                    int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
+                    size_t workspace_bytes, int64_t *offsets, void *stream);
+int tcr_small_fill(const float *x, const float *hx, int64_t nx, const float *y, const float *hy, int64_t ny,
                    int32_t dim, const float *domain_min, const float *domain_max, uint32_t periodic_mask,
                    int32_t wraps, int32_t mode, const double *fixed_support, void *workspace,
                    size_t workspace_bytes, const int64_t *offsets, void *ii, void *jj, int32_t index_bytes,

@@ -81,6 +81,10 @@ def lib():
         L.orc_brute_counts.restype = None
         L.orc_brute_counts.argtypes = [_f32p, _f32p, ctypes.c_int64, _f32p, _f32p, ctypes.c_int64,
                                        ctypes.c_int, _f32p, _f32p, ctypes.c_uint32, ctypes.c_int, _i32p]
+        L.orc_small.restype = None
+        L.orc_small.argtypes = [_f32p, _f32p, ctypes.c_int64, _f32p, _f32p, ctypes.c_int64, ctypes.c_int, _f32p,
+                                _f32p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_float, _i32p, _i64p,
+                                _i64p, _i64p]
         _lib = L
     return _lib
 
@@ -229,3 +233,37 @@ def brute_counts(qpos, qsup, rpos=None, rsup=None, *, domain_min=None, domain_ma
     lib().orc_brute_counts(_p(qpos, _f32p), _p(qsup, _f32p), nq, _p(rpos, _f32p), _p(rsup, _f32p), rpos.shape[0],
                            dim, _p(mn, _f32p), _p(mx, _f32p), mask, MODES[mode], _p(out, _i32p))
     return out
+
+
+def radius_search_small(qpos, qsup, rpos=None, rsup=None, *, domain_min=None, domain_max=None, periodic=None,
+                        mode: str = "gather", support_override: Optional[float] = None):
+    """radiusSearch(..., algorithm='small') as the reference's commented-out
+    op defines it (cpp/neighborhoodSmall_cpu.cpp): default domain [0,1]^d
+    non-periodic; with a domain, positions wrapped once (util.py:375-387);
+    (ii, jj) int64 in (query, reference) order, modDistance < support
+    (fixed: modDistance2 < h*h)."""
+    qpos = _f32(qpos)
+    nq, dim = qpos.shape
+    if rpos is None:
+        rpos, rsup = qpos, qsup
+    rpos = _f32(rpos)
+    if domain_min is None:
+        mn, mx, mask = np.zeros(dim, np.float32), np.ones(dim, np.float32), 0
+    else:
+        mn, mx, mask = _f32(domain_min), _f32(domain_max), _mask(periodic, dim)
+        qpos = wrap(qpos, mn, mx, periodic)
+        rpos = wrap(rpos, mn, mx, periodic)
+    qs = None if qsup is None else _f32(qsup)
+    rs = None if rsup is None else _f32(rsup)
+    fixed = support_override is not None
+    h = float(support_override) if fixed else 0.0
+    args = (_p(qpos, _f32p), _p(qs, _f32p), nq, _p(rpos, _f32p), _p(rs, _f32p), rpos.shape[0], dim,
+            _p(mn, _f32p), _p(mx, _f32p), mask, MODES[mode], int(fixed), ctypes.c_float(h))
+    counts = np.zeros(nq, np.int32)
+    lib().orc_small(*args, _p(counts, _i32p), None, None, None)
+    offs = np.zeros(nq + 1, np.int64)
+    np.cumsum(counts, out=offs[1:])
+    ii = np.zeros(int(offs[-1]), np.int64)
+    jj = np.zeros(int(offs[-1]), np.int64)
+    lib().orc_small(*args, None, _p(offs, _i64p), _p(ii, _i64p), _p(jj, _i64p))
+    return ii, jj

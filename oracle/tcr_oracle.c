@@ -153,11 +153,11 @@ static inline int32_t orc_pymod(int32_t n, int32_t m) { /* cpp/common.h:95-97 */
 
 /* ------------------------------------------------------------- distance --- */
 
-/* modDistance (cpp/common.h:106-129) for float inputs: the fp32 difference,
+/* modDistance2 (cpp/common.h:130-138) for float inputs: the fp32 difference,
  * fp64 periodic modulo, fp64 square added to the fp32 accumulator in fp64 and
- * rounded back to fp32 each step, then sqrtf. */
-static inline float orc_distance(const float *xi, const float *xj, const float *mn,
-                                 const float *mx, uint32_t pmask, int dim) {
+ * rounded back to fp32 each step. */
+static inline float orc_distance2(const float *xi, const float *xj, const float *mn,
+                                  const float *mx, uint32_t pmask, int dim) {
     float sum = 0.0f;
     for (int d = 0; d < dim; ++d) {
         double diff;
@@ -172,7 +172,13 @@ static inline float orc_distance(const float *xi, const float *xj, const float *
         }
         sum = (float)((double)sum + diff * diff);
     }
-    return sqrtf(sum);
+    return sum;
+}
+
+/* modDistance (cpp/common.h:106-129): sqrtf of modDistance2. */
+static inline float orc_distance(const float *xi, const float *xj, const float *mn,
+                                 const float *mx, uint32_t pmask, int dim) {
+    return sqrtf(orc_distance2(xi, xj, mn, mx, pmask, dim));
 }
 
 float orc_pair_distance(const float *xi, const float *xj, const float *mn, const float *mx,
@@ -524,4 +530,40 @@ void orc_brute_counts(const float *qpos, const float *qsup, int64_t nq, const fl
         counts[i] = k;
     }
     free(q); free(r);
+}
+
+/* --------------------------------------------------- small brute force --- */
+
+/* neighborSearchSmall / neighborSearchSmallFixed, the brute-force op the
+ * reference keeps commented out (cpp/neighborhoodSmall_cpu.cpp:45-160,
+ * 162-302): for each query i, every reference j in index order with
+ * modDistance < the mode's support (fixed: modDistance2 < h*h, h*h in fp32).
+ * Positions as given (radiusSearch has wrapped them once).  offsets == NULL:
+ * counts[nq] only; else pairs at offsets[i] into pi (query) / pj (reference). */
+void orc_small(const float *q, const float *hq, int64_t nq, const float *r, const float *hr, int64_t nr,
+               int dim, const float *mn, const float *mx, uint32_t pmask, int mode, int fixed, float h,
+               int32_t *counts, const int64_t *offsets, int64_t *pi, int64_t *pj) {
+    const float h2 = h * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nq; ++i) {
+        int64_t k = offsets ? offsets[i] : 0;
+        int32_t c = 0;
+        for (int64_t j = 0; j < nr; ++j) {
+            int hit;
+            if (fixed) {
+                hit = orc_distance2(q + i * dim, r + j * dim, mn, mx, pmask, dim) < h2;
+            } else {
+                const float dist = orc_distance(q + i * dim, r + j * dim, mn, mx, pmask, dim);
+                hit = orc_pred(dist, hq ? hq[i] : 0.0f, hr ? hr[j] : 0.0f, mode);
+            }
+            if (!hit) continue;
+            if (offsets) {
+                pi[k] = i;
+                pj[k] = j;
+                ++k;
+            }
+            ++c;
+        }
+        if (counts) counts[i] = c;
+    }
 }

@@ -126,3 +126,55 @@ def test_empty_and_errors():
                      algorithm="naive")
     with pytest.raises(ValueError):   # gather without query supports
         radiusSearch(PointCloud(x, None), PointCloud(x, h), mode="gather", algorithm="naive")
+
+
+# ---- algorithm='small': oracle.radius_search_small (C brute force, pinned
+# against the compact oracle and the lattice known answers)
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("fmt", ["coo", "csr"])
+def test_small_vs_oracle(dim, mode, fmt):
+    rng = np.random.default_rng(1300 + dim * 7 + len(mode) + (fmt == "csr"))
+    mn = np.array([-0.4, 0.2, -3.0][:dim], np.float32)
+    mx = np.array([1.3, 0.9, -1.5][:dim], np.float32)
+    ext = mx - mn
+    nx, ny = 1400, 1733
+    x = (mn + rng.uniform(-0.3, 1.3, (nx, dim)) * ext).astype(np.float32)
+    y = (mn + rng.uniform(-0.3, 1.3, (ny, dim)) * ext).astype(np.float32)
+    h = float(ext.min()) * (0.2 if dim == 3 else 0.1)
+    hx = (h * rng.uniform(0.5, 1.5, nx)).astype(np.float32)
+    hy = (h * rng.uniform(0.5, 1.5, ny)).astype(np.float32)
+    per = [bool(v) for v in rng.random(dim) < 0.6]
+    print(f"case: small vs oracle dim={dim} mode={mode} fmt={fmt} per={per}", flush=True)
+    out = radiusSearch(PointCloud(_t(x), _t(hx)), PointCloud(_t(y), _t(hy)), mode=mode, domain=_dom(mn, mx, per),
+                       algorithm="small", format=fmt)
+    ii, jj = oracle.radius_search_small(x, hx, y, hy, domain_min=mn, domain_max=mx, periodic=per, mode=mode)
+    assert ii.shape[0] > 1000
+    if fmt == "coo":
+        np.testing.assert_array_equal(out.row.cpu().numpy(), ii)
+        np.testing.assert_array_equal(out.col.cpu().numpy(), jj)
+    else:
+        nj = np.bincount(ii, minlength=nx)
+        np.testing.assert_array_equal(out.indices.cpu().numpy(), jj)
+        np.testing.assert_array_equal(out.rowEntries.cpu().numpy(), nj)
+        np.testing.assert_array_equal(out.indptr.cpu().numpy(), np.concatenate([[0], np.cumsum(nj)]).astype(np.int32))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_small_known_answers_and_fixed(periodic):
+    Y, hY, P, hP, mn, mx = reference_test_data()
+    dom = _dom(mn, mx, [periodic] * 2)
+    print(f"case: small lattice periodic={periodic}", flush=True)
+    for override in (None, float(hP[0])):
+        sup = None if override is not None else _t(hP)
+        full = radiusSearch(PointCloud(_t(P), sup), None, supportOverride=override, domain=dom, algorithm="small")
+        counts = torch.bincount(full.row, minlength=P.shape[0]).cpu().numpy()
+        if periodic:
+            assert full.row.shape[0] == 46080 and counts.min() == counts.max() == 45
+        else:
+            assert full.row.shape[0] == 41580 and counts.min() == 15 and counts.max() == 45
+        ii, jj = oracle.radius_search_small(P, hP, None, None, domain_min=mn, domain_max=mx,
+                                            periodic=[periodic] * 2, mode="gather", support_override=override)
+        np.testing.assert_array_equal(full.row.cpu().numpy(), ii)
+        np.testing.assert_array_equal(full.col.cpu().numpy(), jj)

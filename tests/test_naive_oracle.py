@@ -97,3 +97,49 @@ def test_mod_quirk_closed_forms():
     y = np.linspace(-1.999, 1.999, 4001).astype(np.float32)
     want = np.where(np.abs(y) > a, y - np.sign(y) * np.float32(2) * a, y).astype(np.float32)
     np.testing.assert_array_equal(oracle.naive_mod(y, -1.0, 1.0), want)
+
+
+# ---- algorithm='small' (cpp/neighborhoodSmall_cpu.cpp, commented out of the
+# reference build): oracle.radius_search_small, the C brute force over the
+# compact path's distance and predicate.
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_small_matches_compact_oracle(dim, mode, periodic):
+    """Same distance and predicate as the compact oracle; inside the domain a
+    once-wrapped reference point is its twice-wrapped self, so the pair sets
+    agree exactly; the order is (query, reference)."""
+    rng = np.random.default_rng(700 + dim * 5 + periodic)
+    q = rng.uniform(-0.999, 0.999, (500, dim)).astype(np.float32)
+    r = rng.uniform(-0.999, 0.999, (620, dim)).astype(np.float32)
+    h = 0.25 if dim < 3 else 0.35
+    hq = (h * rng.uniform(0.6, 1.4, 500)).astype(np.float32)
+    hr = (h * rng.uniform(0.6, 1.4, 620)).astype(np.float32)
+    mn, mx = np.full(dim, -1, np.float32), np.full(dim, 1, np.float32)
+    per = [periodic] * dim
+    ii, jj = oracle.radius_search_small(q, hq, r, hr, domain_min=mn, domain_max=mx, periodic=per, mode=mode)
+    key = ii * (r.shape[0] + 1) + jj
+    assert np.all(np.diff(key) > 0)
+    o = oracle.radius_search(q, hq, r, hr, domain_min=mn, domain_max=mx, periodic=per, mode=mode)
+    assert ii.shape[0] > 1000
+    assert set(zip(ii.tolist(), jj.tolist())) == set(zip(o.row.tolist(), o.col.tolist()))
+    np.testing.assert_array_equal(np.bincount(ii, minlength=q.shape[0]), o.counts)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_small_reference_known_answers(periodic):
+    """scripts/test.py:126-137 asserts its lattice counts for 'small' with
+    per-point and with fixed supports."""
+    Y, hY, P, hP, mn, mx = reference_test_data()
+    for fixed in (None, float(hP[0])):
+        ii, _ = oracle.radius_search_small(P, hP, P, hP, domain_min=mn, domain_max=mx, periodic=[periodic] * 2,
+                                           mode="gather", support_override=fixed)
+        counts = np.bincount(ii, minlength=P.shape[0])
+        if periodic:
+            assert ii.shape[0] == 46080 and counts.min() == counts.max() == 45
+        else:
+            assert ii.shape[0] == 41580 and counts.min() == 15 and counts.max() == 45
+        ii, _ = oracle.radius_search_small(Y, hY, P, hP, domain_min=mn, domain_max=mx, periodic=[periodic] * 2,
+                                           mode="gather", support_override=fixed)
+        assert ii.shape[0] == 11520

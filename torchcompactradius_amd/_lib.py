@@ -62,7 +62,7 @@ EXPORTED = (
     "tcr_graph_capture", "tcr_graph_launch", "tcr_graph_destroy", "tcr_set_tuning", "tcr_get_tuning",
     "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill", "tcr_csr_arrays",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
-    "tcr_naive_workspace_size", "tcr_naive_count", "tcr_naive_fill",
+    "tcr_naive_workspace_size", "tcr_naive_count", "tcr_naive_fill", "tcr_small_count", "tcr_small_fill",
     "tcr_last_error", "tcr_version",
 )
 
@@ -133,12 +133,14 @@ def load() -> ctypes.CDLL:
     dp = ctypes.POINTER(ctypes.c_double)
     L.tcr_naive_workspace_size.restype = ctypes.c_size_t
     L.tcr_naive_workspace_size.argtypes = [i64, i64, i32]
-    L.tcr_naive_count.restype = ctypes.c_int
-    L.tcr_naive_count.argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, ctypes.c_uint32, i32, i32, dp, vp,
-                                  ctypes.c_size_t, vp, vp]
-    L.tcr_naive_fill.restype = ctypes.c_int
-    L.tcr_naive_fill.argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, ctypes.c_uint32, i32, i32, dp, vp,
-                                 ctypes.c_size_t, vp, vp, vp, i32, vp]
+    for name in ("tcr_naive_count", "tcr_small_count"):
+        getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, ctypes.c_uint32, i32, i32, dp, vp,
+                                     ctypes.c_size_t, vp, vp]
+    for name in ("tcr_naive_fill", "tcr_small_fill"):
+        getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, ctypes.c_uint32, i32, i32, dp, vp,
+                                     ctypes.c_size_t, vp, vp, vp, i32, vp]
     _lib = L
     return L
 

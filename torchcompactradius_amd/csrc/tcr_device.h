@@ -106,12 +106,11 @@ __device__ __forceinline__ int32_t pymod(int32_t n, int32_t m) {  // cpp/common.
     return n >= 0 ? n % m : ((n % m) + m) % m;
 }
 
-// modDistance (cpp/common.h:106-129) for float positions: fp32 difference,
-// fp64 periodic modulo, fp64 square accumulated into an fp32 sum, sqrtf.
-// exact.
+// modDistance2 (cpp/common.h:130-138) for float positions: fp32 difference,
+// fp64 periodic modulo, fp64 square accumulated into an fp32 sum.  exact.
 template <int DIM>
-__device__ __forceinline__ float exact_distance(const float *xi, const float *xj,
-                                                const float *mn, const float *mx, uint32_t pmask) {
+__device__ __forceinline__ float exact_distance2(const float *xi, const float *xj,
+                                                 const float *mn, const float *mx, uint32_t pmask) {
     float sum = 0.0f;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -126,7 +125,14 @@ __device__ __forceinline__ float exact_distance(const float *xi, const float *xj
         }
         sum = (float)((double)sum + diff * diff);
     }
-    return sqrtf(sum);
+    return sum;
+}
+
+// modDistance (cpp/common.h:106-129): sqrtf of modDistance2.  exact.
+template <int DIM>
+__device__ __forceinline__ float exact_distance(const float *xi, const float *xj,
+                                                const float *mn, const float *mx, uint32_t pmask) {
+    return sqrtf(exact_distance2<DIM>(xi, xj, mn, mx, pmask));
 }
 
 // std::max(a, b) as the reference calls it: (a < b) ? b : a.

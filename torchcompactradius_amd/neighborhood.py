@@ -54,14 +54,17 @@ def radiusSearch(
     indices, columns reference indices, row-sorted; within a row the order is
     the reference compact path's (cell stencil order, then original index).
 
-    ``algorithm='compact'``, ``'mlm'`` and ``'naive'`` are computed by this
-    package, every search on the GPU; ``'naive'`` keeps the reference's
-    brute-force semantics (``<=``, its own periodic ``mod``, pairs ordered by
-    reference index).  ``'mlm'`` -- the multi-level search for variable
-    supports -- returns exactly the compact result (pairs and order); for
-    inputs it does not take (gather mode, points outside the grid, ...) and
-    with ``returnStructure`` it runs the compact search.  ``supportOverride`` maps a scalar support onto every
-    point (the reference's fixed-support compact variant is not callable).
+    Every algorithm runs on the GPU.  ``'naive'`` (the default) keeps the
+    reference's brute-force semantics (``<=``, its own periodic ``mod``, pairs
+    ordered by reference index); ``'small'`` is the brute-force op the
+    reference keeps commented out (compact distance and predicate, pairs in
+    (query, reference) order).  ``'cluster'`` is not accepted, as in the
+    reference without torch_cluster.  ``'mlm'`` -- the multi-level search for
+    variable supports -- returns exactly the compact result (pairs and order);
+    for inputs it does not take (gather mode, points outside the grid, ...)
+    and with ``returnStructure`` it runs the compact search.  For 'compact'
+    and 'mlm', ``supportOverride`` maps a scalar support onto every point (the
+    reference's fixed-support compact variant is not callable).
     ``indexDtype`` (keyword only, an extension) selects int64 row/col
     indices -- the reference's -- or int32 ones.
     """
@@ -85,12 +88,9 @@ def radiusSearch(
         if referencePointCloud is not None and referencePointCloud.supports is not None:
             assert referencePointCloud.supports.shape[0] == referencePointCloud.positions.shape[0], \
                 "support.shape[0] != referencePositions.shape[0]"
-        if algorithm == "naive":
-            return _naive(queryPointCloud, referencePointCloud, supportOverride, mode, domain, verbose, format,
-                          returnStructure, indexDtype)
-        if algorithm not in ("compact", "mlm"):
-            raise NotImplementedError(
-                f"algorithm='{algorithm}' is not provided by torchcompactradius_amd; use 'compact', 'mlm' or 'naive'")
+        if algorithm in ("naive", "small"):
+            return _brute(algorithm == "small", queryPointCloud, referencePointCloud, supportOverride, mode, domain,
+                          verbose, format, returnStructure, indexDtype)
 
         qpos = queryPointCloud.positions
         qsup = queryPointCloud.supports
@@ -131,19 +131,26 @@ def radiusSearch(
         return out
 
 
-def _naive(queryPointCloud: PointCloud, referencePointCloud: Optional[PointCloud], supportOverride: Optional[float],
-           mode: str, domain: Optional[DomainDescription], verbose: bool, format: str, returnStructure: bool,
-           indexDtype: torch.dtype):
-    """radiusSearch's algorithm='naive' branch (neighborhood.py:186-271,
-    298-309): with a domain, periodic coordinates are wrapped once here
-    (getPeriodicPointCloud) and once more by radiusNaive; without one the
-    default domain [0,1]^d is not periodic.  The pairs come in radiusNaive's
-    order -- by reference index, then query index -- and the CSR form is
+def _brute(small: bool, queryPointCloud: PointCloud, referencePointCloud: Optional[PointCloud],
+           supportOverride: Optional[float], mode: str, domain: Optional[DomainDescription], verbose: bool,
+           format: str, returnStructure: bool, indexDtype: torch.dtype):
+    """radiusSearch's brute-force branches (neighborhood.py:186-279, 298-309);
+    without a domain the default [0,1]^d is not periodic, and with one the
+    periodic coordinates are wrapped once here (getPeriodicPointCloud).
+
+    'naive' (radiusNaive.py:8-56): radiusNaive wraps them once more; pairs in
+    its order -- by reference index, then query index -- and the CSR form is
     coo_to_csr(isSorted=True) of them, as the reference builds it (its indices
-    are therefore in that order too).  supportOverride selects radiusNaiveFixed
-    (squared distance <= support^2), the branch the reference dispatches to
-    (:226-229) but cannot run (it passes PointCloud records where the scripted
-    function takes tensors)."""
+    in that order too).  supportOverride selects radiusNaiveFixed (squared
+    distance <= support^2), which the reference dispatches to (:226-229) but
+    cannot run (it passes PointCloud records where the scripted function takes
+    tensors).
+
+    'small' (cpp/neighborhoodSmall_cpu.cpp:45-302, commented out of the
+    reference build, whose binding is None -- the reference raises there): the
+    compact path's distance and strict predicate over all pairs, in (query,
+    reference) order; supportOverride selects its fixed variant (modDistance2 <
+    support^2 in fp32)."""
     from .search import _f32, run_naive
     qpos = _f32(queryPointCloud.positions, "queryPointCloud.positions")
     qsup = _f32(queryPointCloud.supports, "queryPointCloud.supports")
@@ -154,10 +161,11 @@ def _naive(queryPointCloud: PointCloud, referencePointCloud: Optional[PointCloud
         rsup = _f32(referencePointCloud.supports, "referencePointCloud.supports")
     dim = qpos.shape[1]
     mn, mx, pmask = _domain(domain, dim)
-    wraps = 2 if domain is not None else 0
+    wraps = 0 if domain is None else (1 if small else 2)
     if verbose:
-        print("Calling radiusNaiveFixed:" if supportOverride is not None else "Calling radiusNaive")
-    ii, jj = run_naive(qpos, qsup, rpos, rsup, mn, mx, pmask, wraps, mode, supportOverride, indexDtype)
+        name = "neighborSearchSmall" if small else "radiusNaive"
+        print(f"Calling {name}{'Fixed' if supportOverride is not None else ''}")
+    ii, jj = run_naive(qpos, qsup, rpos, rsup, mn, mx, pmask, wraps, mode, supportOverride, indexDtype, small)
     out: Union[SparseCOO, SparseCSR] = SparseCOO(ii, jj, qpos.shape[0], rpos.shape[0])
     if format == "csr":
         out = coo_to_csr(out, isSorted=True)

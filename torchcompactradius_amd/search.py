@@ -397,10 +397,12 @@ def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_level
 
 def run_naive(qpos: torch.Tensor, qsup: Optional[torch.Tensor], rpos: torch.Tensor, rsup: Optional[torch.Tensor],
               domain_min: Sequence[float], domain_max: Sequence[float], periodic: int, wraps: int, mode: str,
-              fixed_support: Optional[float] = None, index_dtype=torch.int64):
-    """Brute-force search with the semantics of algorithm='naive'
-    (searchAlgorithms/radiusNaive.py:8-56) on libtcr_hip.so: (ii, jj) query
-    and reference indices ordered by reference index, then query index."""
+              fixed_support: Optional[float] = None, index_dtype=torch.int64, small: bool = False):
+    """Brute-force search on libtcr_hip.so: (ii, jj) query and reference
+    indices.  small=False: algorithm='naive' (searchAlgorithms/radiusNaive.py:
+    8-56), ordered by reference index, then query index; small=True:
+    algorithm='small' (cpp/neighborhoodSmall_cpu.cpp, commented out of the
+    reference build), ordered by query index, then reference index."""
     if index_dtype not in (torch.int64, torch.int32):
         raise ValueError("index_dtype must be torch.int64 or torch.int32")
     L = _lib.load()
@@ -417,17 +419,17 @@ def run_naive(qpos: torch.Tensor, qsup: Optional[torch.Tensor], rpos: torch.Tens
     if ws_bytes == 0:
         check(_lib.TCR_ERR_ARG)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
-    offsets = torch.empty(ny + 1, dtype=torch.int64, device=device)
+    offsets = torch.empty((nx if small else ny) + 1, dtype=torch.int64, device=device)
     stream = _stream()
     args = (ptr(qpos), ptr(qsup), nx, ptr(rpos), ptr(rsup), ny, dim, mn, mx, int(periodic), int(wraps),
             MODES[mode], fixed, ptr(ws), ws_bytes)
-    check(L.tcr_naive_count(*args, ptr(offsets), stream))
+    count, fill = (L.tcr_small_count, L.tcr_small_fill) if small else (L.tcr_naive_count, L.tcr_naive_fill)
+    check(count(*args, ptr(offsets), stream))
     total = int(offsets[-1].item())
     ii = torch.empty(total, dtype=index_dtype, device=device)
     jj = torch.empty(total, dtype=index_dtype, device=device)
     if total > 0:
-        check(L.tcr_naive_fill(*args, ptr(offsets), ptr(ii), ptr(jj), 8 if index_dtype == torch.int64 else 4,
-                               stream))
+        check(fill(*args, ptr(offsets), ptr(ii), ptr(jj), 8 if index_dtype == torch.int64 else 4, stream))
     return ii, jj
 
 
