@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-support multi-level search")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--only", choices=("batched", "served", "adaptive"),
+                    help="run only this comparison leg and print its JSON object (to profile it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the search does not "
@@ -303,6 +305,10 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.only:
+        leg = {"batched": bench_batched, "served": bench_served, "adaptive": bench_adaptive}[args.only]
+        print(json.dumps({"leg": args.only, **leg(torch)}), flush=True)
+        return
     from torchcompactradius_amd import DomainDescription, PointCloud, _lib, radiusSearch, synthetic
 
     w = synthetic.periodic_3d(n=args.n)
