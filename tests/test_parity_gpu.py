@@ -142,6 +142,29 @@ def test_small_periodic_grids_visit_cells_twice():
     assert_same(g, o)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("ncells", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("mode", ["gather", "symmetric", "superSymmetric"])
+def test_few_cells_in_periodic_dims(dim, ncells, mode):
+    """Periodic dimensions only a few cells wide: candidates reach the query
+    through the image across the boundary, and the cell-centred estimate has
+    to pick the reference's minimal image."""
+    rng = np.random.default_rng(40 + dim * 10 + ncells + len(mode))
+    n = 2500 if dim == 3 else 1200
+    q = rng.uniform(0, 1, (n, dim)).astype(np.float32)
+    q[:60, 0] = rng.uniform(0, 0.02, 60).astype(np.float32)      # hugging both faces
+    q[60:120, 0] = rng.uniform(0.98, 1.0, 60).astype(np.float32)
+    hmax = 0.999 / ncells
+    h = (hmax * rng.uniform(0.6, 1.0, n)).astype(np.float32)
+    h[0] = np.float32(hmax)
+    per = [True, False, True][:dim] if dim == 3 else [True, True]
+    print(f"case: few periodic cells dim={dim} ncells={ncells} mode={mode}", flush=True)
+    g = gpu_search(q, h, mn=[0] * dim, mx=[1] * dim, periodic=per, mode=mode, L=1024)
+    o = oracle.radius_search(q, h, domain_min=[0] * dim, domain_max=[1] * dim, periodic=per, mode=mode,
+                             hash_map_length=1024)
+    assert_same(g, o)
+
+
 @pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_clustered_big_cells_and_member_batches(mode, periodic):

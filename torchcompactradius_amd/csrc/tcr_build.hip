@@ -56,15 +56,19 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
     cd.hcell = hc;
     uint32_t err = 0;
     // wavefront-per-cell eligibility: self-search, stencil radius 1, a
-    // finite positive cell size, and >= 6 cells in every periodic dimension
-    // (so that every candidate in the stencil box has one minimal image)
+    // finite positive cell size, and >= 5 cells in every periodic dimension.
+    // A pair within support (<= hCell) has its minimal image within 1.5 cells
+    // of the query's cell centre, and nc = ceil(ext / hCell) >= 5 gives
+    // ext > 4 cells: the image nearest the centre (|offset| <= ext / 2 > 2
+    // cells) is that minimal image, with half a cell to spare for rounding,
+    // and the 3 stencil cells of a dimension are distinct.
     uint32_t fast = (cd.h.self_search && cd.h.search_range == 1 && hc > 0.0f && !isinf(hc)) ? 1u : 0u;
     for (int d = 0; d < 3; ++d) {
         if (d < dim) {
             const float v = ceilf(ext[d] / hc);
             cd.nc[d] = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)v : INT32_MIN;
             if (((cd.h.periodic_mask >> d) & 1u) && cd.nc[d] <= 0 && cd.h.nq > 0 && cd.h.nr > 0) err = 1;
-            if (cd.nc[d] <= 0 || (((cd.h.periodic_mask >> d) & 1u) && cd.nc[d] < 6)) fast = 0;
+            if (cd.nc[d] <= 0 || (((cd.h.periodic_mask >> d) & 1u) && cd.nc[d] < 5)) fast = 0;
         } else {
             cd.nc[d] = 1;
         }

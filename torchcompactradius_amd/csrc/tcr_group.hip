@@ -179,7 +179,7 @@ __device__ __forceinline__ bool cell_coords(const CloudDev &cd, float4 p0, int32
 }
 
 // All 3^DIM stencil cells of one cell: the periodic neighbours of an in-grid
-// coordinate are q - 1 + n or q + 1 - n (a fast cloud has >= 6 cells in every
+// coordinate are q - 1 + n or q + 1 - n (a fast cloud has >= 5 cells in every
 // periodic dimension).  In a cloud whose points all lie in the grid, a
 // neighbour whose coordinates all lie in the grid is its class's own cell,
 // whose bucket is hash(c); only coordinates outside the grid (non-periodic
@@ -410,7 +410,7 @@ struct Geo {
     }
     // place() for a cell none of whose stencil neighbours lies across a
     // periodic boundary: a candidate of such a stencil is within 1.5 cells of
-    // the centre and a periodic extent spans >= 6 cells, so no image shift
+    // the centre and a periodic extent spans > 4 cells, so no image shift
     // applies (an aliased candidate from elsewhere fails the box test as in
     // place() and is decided exactly)
     __device__ __forceinline__ void place_direct(float4 v, bool valid, float *xc, float &p) const {
