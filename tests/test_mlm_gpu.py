@@ -119,3 +119,30 @@ def test_non_self_search():
                                  mode=mode, hash_map_length=4096)
         np.testing.assert_array_equal(a.row.cpu().numpy(), o.row)
         np.testing.assert_array_equal(a.col.cpu().numpy(), o.col)
+
+
+@pytest.mark.parametrize("mode", ["scatter", "symmetric", "superSymmetric"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_distances_at_the_support(mode, dim):
+    """A lattice whose pair distances equal the supports or lie a rounding
+    step away from them: the fp32 estimate of a cloud without periodic
+    dimensions leaves these to the reference formula, which must match the
+    oracle bit for bit."""
+    rng = np.random.default_rng(880 + dim + len(mode))
+    g = np.arange(-7, 8, dtype=np.float32) * np.float32(0.125)       # exact in fp32
+    grid = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), axis=-1).reshape(-1, dim).astype(np.float32)
+    # supports exactly lattice distances (1, sqrt(2), 2 spacings) and their fp32 neighbours
+    cand = np.array([0.125, 0.25, np.sqrt(2) * 0.125, 0.375], np.float32)
+    sup = cand[rng.integers(0, cand.size, grid.shape[0])]
+    step = rng.integers(-1, 2, grid.shape[0])
+    sup = np.where(step < 0, np.nextafter(sup, np.float32(0)), np.where(step > 0, np.nextafter(sup, np.float32(1)), sup))
+    sup = sup.astype(np.float32)
+    mn, mx = [-1.0] * dim, [1.0] * dim
+    per = [False] * dim
+    print(f"case: mlm support ties mode={mode} dim={dim}", flush=True)
+    r = run_mlm(_spec(grid, sup, mn, mx, per, mode), want_rows=True)
+    torch.cuda.synchronize()
+    o = oracle.radius_search(grid, sup, domain_min=mn, domain_max=mx, periodic=per, mode=mode, hash_map_length=4096)
+    assert o.row.shape[0] > 1000
+    np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+    np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
