@@ -146,3 +146,26 @@ def test_distances_at_the_support(mode, dim):
     assert o.row.shape[0] > 1000
     np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
     np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+
+
+@pytest.mark.parametrize("cap", [1, 4, 2048])
+def test_slot_capacity_and_overflow(cap, monkeypatch):
+    """The count pass keeps each row's keys in a slot of MLM_SLOT_CAP; rows
+    longer than it send the fill back to a second traversal.  Either way the
+    output is the oracle's."""
+    import torchcompactradius_amd.search as S
+    monkeypatch.setattr(S, "MLM_SLOT_CAP", cap)
+    rng = np.random.default_rng(990 + cap)
+    pos = rng.uniform(-1, 1, (3000, 3)).astype(np.float32)
+    sup = (0.06 * np.exp2(rng.uniform(0, 3, 3000))).astype(np.float32)
+    mn, mx, per = [-1.0] * 3, [1.0] * 3, [False] * 3
+    print(f"case: mlm slot cap={cap}", flush=True)
+    for fmt_rows, dt in ((True, torch.int64), (False, torch.int32)):
+        r = run_mlm(_spec(pos, sup, mn, mx, per, "symmetric"), want_rows=fmt_rows, index_dtype=dt)
+        torch.cuda.synchronize()
+        o = oracle.radius_search(pos, sup, domain_min=mn, domain_max=mx, periodic=per, mode="symmetric",
+                                 hash_map_length=4096)
+        assert o.counts.max() > 4
+        if fmt_rows:
+            np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(r.col.cpu().numpy().astype(np.int64), o.col)

@@ -109,9 +109,9 @@ def load() -> ctypes.CDLL:
     L.tcr_mlm_workspace_size.argtypes = [cl]
     L.tcr_mlm_count.restype = ctypes.c_int
     L.tcr_mlm_count.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.POINTER(ctypes.c_int64),
-                                ctypes.POINTER(ctypes.c_int32), vp]
+                                ctypes.POINTER(ctypes.c_int32), vp, i64, ctypes.POINTER(ctypes.c_int32), vp]
     L.tcr_mlm_fill.restype = ctypes.c_int
-    L.tcr_mlm_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp, vp]
+    L.tcr_mlm_fill.argtypes = [cl, i32, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, vp, vp, i64, vp]
     L.tcr_csr_arrays.restype = ctypes.c_int
     L.tcr_csr_arrays.argtypes = [vp, i64, vp, vp, vp]
     L.tcr_timing_enable.restype = ctypes.c_int

@@ -561,10 +561,15 @@ size_t tcr_mlm_workspace_size(const tcr_cloud *cloud) {
 
 int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos, const float *qsup,
                   const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes, int64_t *offsets,
-                  int64_t *total_pairs, int32_t *levels_used, void *stream) {
+                  int64_t *total_pairs, int32_t *levels_used, uint64_t *slots, int64_t slot_cap,
+                  int32_t *slots_complete, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (!cloud || !offsets) {
         set_error("cloud or offsets missing");
+        return TCR_ERR_ARG;
+    }
+    if (slots && (slot_cap < 1 || slot_cap > kMlmSlotMax)) {
+        set_error("slot_cap must be 1.." + std::to_string(kMlmSlotMax));
         return TCR_ERR_ARG;
     }
     if (max_levels < 1 || max_levels > 4) {
@@ -613,7 +618,7 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
     if (rc) return rc;
     rc = mlm_groups_launch(p.b, p.w, s);
     if (rc) return rc;
-    rc = mlm_query_launch(p.b, p.w, L, false, nullptr, nullptr, s);
+    rc = mlm_query_launch(p.b, p.w, L, false, nullptr, nullptr, s, slots, slots ? slot_cap : 0);
     if (rc) return rc;
     rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
     if (rc) return rc;
@@ -626,7 +631,8 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
                        (const uint32_t *)(p.w.misc + kMiscMlmFlags), rb);
     TCR_LAUNCH_CHECK();
     TCR_HIP(hipStreamSynchronize(s));
-    if (rb->err) {
+    if (slots_complete) *slots_complete = slots && !(rb->err & 8u) ? 1 : 0;  // 8: a row outgrew its slot
+    if (rb->err & 7u) {
         set_error(std::string("multi-level search does not take these inputs:") +
                   ((rb->err & 1) ? " non-finite or negative supports" : "") +
                   ((rb->err & 2) ? " reference points outside the grid" : "") +
@@ -640,12 +646,17 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
 
 int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels, const float *qpos, const float *qsup, const float *rpos,
                  const float *rsup, void *workspace, size_t workspace_bytes, const int64_t *offsets, void *row,
-                 void *col, int32_t index_bytes, uint64_t *scratch, void *stream) {
+                 void *col, int32_t index_bytes, uint64_t *scratch, const uint64_t *slots, int64_t slot_cap,
+                 void *stream) {
     hipStream_t s = (hipStream_t)stream;
     PairOut out;
     if (!pair_out(row, col, index_bytes, out)) return TCR_ERR_ARG;
-    if (!cloud || !offsets || !col || !scratch) {
+    if (!cloud || !offsets || !col || (!scratch && !slots)) {
         set_error("cloud, offsets, col or scratch missing");
+        return TCR_ERR_ARG;
+    }
+    if (slots && (slot_cap < 1 || slot_cap > kMlmSlotMax)) {
+        set_error("slot_cap must be 1.." + std::to_string(kMlmSlotMax));
         return TCR_ERR_ARG;
     }
     if (levels < 1 || levels > 4) {
@@ -655,6 +666,8 @@ int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels, const float *qpos, cons
     Prepared p;
     int rc = prepare(cloud, 1, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false, true, 4);
     if (rc) return rc;
+    if (slots)  // every row is in its slot already: sort from there
+        return mlm_sort_rows_launch(p.b, offsets, const_cast<uint64_t *>(slots), out, s, slot_cap);
     rc = mlm_query_launch(p.b, p.w, levels, true, offsets, scratch, s);
     if (rc) return rc;
     return mlm_sort_rows_launch(p.b, offsets, scratch, out, s);

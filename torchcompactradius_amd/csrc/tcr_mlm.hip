@@ -49,7 +49,7 @@ constexpr int kMlmLevelShift = 29;
 constexpr uint32_t kMlmGroupBit = 0x80000000u;
 constexpr int kMlmSortMax = 2048;           // rows up to this length sort in LDS (16 KiB per wavefront)
 
-enum : uint32_t { kMlmBadSupport = 1, kMlmOutOfGrid = 2, kMlmQueryOutOfGrid = 4 };
+enum : uint32_t { kMlmBadSupport = 1, kMlmOutOfGrid = 2, kMlmQueryOutOfGrid = 4, kMlmSlotOverflow = 8 };
 
 __device__ __forceinline__ float level_cell(float hcell, int l) { return ldexpf(hcell, -l); }
 
@@ -252,6 +252,8 @@ struct MlmArgs {
     int32_t *counts;
     const int64_t *offsets;
     uint64_t *scratch;   // fill: (stencil offset << 32 | original index) per hit
+    uint64_t *slots;     // emit: the same keys, query g's at [g * slot_cap, (g + 1) * slot_cap)
+    int64_t slot_cap;
 };
 
 // Pair support bound for a query of support hi against any point of a
@@ -264,8 +266,14 @@ __device__ __forceinline__ float level_radius(int mode, float hi, float hl) {
     }
 }
 
-template <int DIM, bool FILL>
+// Passes of the traversal: count the hits; fill their keys at the row's
+// offset; or emit -- count them and write their keys into the query's slot
+// (rows longer than a slot raise kMlmSlotOverflow).
+enum : int { kMlmCount = 0, kMlmFill = 1, kMlmEmit = 2 };
+
+template <int DIM, int PASS>
 __device__ void mlm_query(const MlmArgs &a, int64_t g) {
+    constexpr bool FILL = PASS == kMlmFill;
     constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
     const CloudDev &cd = a.desc[0];
     float xi[3] = {0.f, 0.f, 0.f}, mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
@@ -301,7 +309,9 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
         const float xj[3] = {v.x, v.y, v.z};
         const float dist = exact_distance<DIM>(xi, xj, mn, mx, pmask);
         if (!support_pred(dist, hi, v.w, mode)) return;
-        if (FILL) a.scratch[base + k] = ((uint64_t)(uint32_t)o << 32) | (uint32_t)a.sort_index[j];
+        const uint64_t key = ((uint64_t)(uint32_t)o << 32) | (uint32_t)a.sort_index[j];
+        if (FILL) a.scratch[base + k] = key;
+        if (PASS == kMlmEmit && k < a.slot_cap) a.slots[g * a.slot_cap + k] = key;
         ++k;
     };
 
@@ -368,25 +378,26 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
         }
     }
     if (!FILL) a.counts[g] = (int32_t)k;
+    if (PASS == kMlmEmit && k > a.slot_cap) atomicOr(a.flags, kMlmSlotOverflow);
 }
 
 // One thread per query.  A self-search takes its queries in sorted order
 // (thread t: the point at sorted slot t), so that the lanes of a wavefront
 // hold neighbouring points of one level and walk the same cells.
-template <bool FILL>
+template <int PASS>
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_query(MlmArgs a) {
     const int64_t t = (int64_t)blockIdx.x * kMlmBlock + threadIdx.x;
     if (t >= a.nq) return;
     const int64_t g = a.desc[0].h.self_search ? (int64_t)a.sort_index[t] : t;
     switch (a.desc[0].h.dim) {
-        case 1: mlm_query<1, FILL>(a, g); break;
-        case 2: mlm_query<2, FILL>(a, g); break;
-        default: mlm_query<3, FILL>(a, g); break;
+        case 1: mlm_query<1, PASS>(a, g); break;
+        case 2: mlm_query<2, PASS>(a, g); break;
+        default: mlm_query<3, PASS>(a, g); break;
     }
 }
 
 int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64_t *offsets, uint64_t *scratch,
-                     hipStream_t s) {
+                     hipStream_t s, uint64_t *slots, int64_t slot_cap) {
     if (b.nq_total == 0) return TCR_OK;
     MlmArgs a;
     a.desc = w.desc;
@@ -402,9 +413,12 @@ int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64
     a.counts = w.counts;
     a.offsets = offsets;
     a.scratch = scratch;
+    a.slots = slots;
+    a.slot_cap = slot_cap;
     const dim3 grid((unsigned)((b.nq_total + kMlmBlock - 1) / kMlmBlock)), block(kMlmBlock);
-    if (fill) hipLaunchKernelGGL(k_mlm_query<true>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_mlm_query<false>, grid, block, 0, s, a);
+    if (fill) hipLaunchKernelGGL(k_mlm_query<kMlmFill>, grid, block, 0, s, a);
+    else if (slots) hipLaunchKernelGGL(k_mlm_query<kMlmEmit>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_mlm_query<kMlmCount>, grid, block, 0, s, a);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
@@ -422,11 +436,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // longer rows, up to kMlmSortMax in LDS (bitonic network over the next power
 // of two, padded with all-ones keys), and has lane 0 sort still longer rows
 // in place (insertion sort).  `scratch` may alias the int64 column output:
-// every entry is read before it is overwritten.
-__device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t m, const uint64_t *scratch,
-                                               const PairOut &out) {
+// every entry is read before it is overwritten.  With src_cap > 0 the keys
+// come from the emit pass's slots (query g's at g * src_cap), not from the
+// row's own offset; such rows are at most src_cap <= kMlmSortMax long.
+__device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t src, int64_t m,
+                                               const uint64_t *scratch, const PairOut &out) {
     const int lane = __lane_id();
-    uint64_t key = lane < m ? scratch[b + lane] : ~0ull;
+    uint64_t key = lane < m ? scratch[src + lane] : ~0ull;
     // a network over the next power of two >= m: lanes beyond it never pair
     // with lanes below it
 #pragma unroll
@@ -447,17 +463,18 @@ __device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t m, 
 }
 
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_short(int64_t nq, const int64_t *__restrict__ offsets,
-                                                              const uint64_t *scratch, PairOut out) {
+                                                              const uint64_t *scratch, int64_t src_cap,
+                                                              PairOut out) {
     const int64_t wid = ((int64_t)blockIdx.x * kMlmBlock + threadIdx.x) / kWave;
     const int64_t nw = ((int64_t)gridDim.x * kMlmBlock) / kWave;
     for (int64_t g = wid; g < nq; g += nw) {
         const int64_t b = offsets[g], m = offsets[g + 1] - b;
-        if (m > 0 && m <= kWave) sort_row_short(g, b, m, scratch, out);
+        if (m > 0 && m <= kWave) sort_row_short(g, b, src_cap > 0 ? g * src_cap : b, m, scratch, out);
     }
 }
 
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const int64_t *__restrict__ offsets,
-                                                             uint64_t *scratch, PairOut out) {
+                                                             uint64_t *scratch, int64_t src_cap, PairOut out) {
     __shared__ uint64_t s_keys[kMlmBlock / kWave][kMlmSortMax];
     uint64_t *sk = s_keys[threadIdx.x / kWave];
     const int lane = threadIdx.x & (kWave - 1);
@@ -466,10 +483,11 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const i
     for (int64_t g = wid; g < nq; g += nw) {
         const int64_t b = offsets[g], m = offsets[g + 1] - b;
         if (m <= kWave) continue;  // k_mlm_sort_short
+        const int64_t src = src_cap > 0 ? g * src_cap : b;
         if (m <= kMlmSortMax) {
             int n2 = 1;
             while (n2 < m) n2 <<= 1;
-            for (int t = lane; t < n2; t += kWave) sk[t] = t < m ? scratch[b + t] : ~0ull;
+            for (int t = lane; t < n2; t += kWave) sk[t] = t < m ? scratch[src + t] : ~0ull;
             wave_lds_sync();
             for (int kk = 2; kk <= n2; kk <<= 1)
                 for (int j = kk >> 1; j > 0; j >>= 1) {
@@ -501,14 +519,19 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const i
     }
 }
 
-int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s) {
+int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s,
+                         int64_t src_cap) {
     if (b.nq_total == 0) return TCR_OK;
+    if (src_cap > kMlmSortMax) {
+        set_error("slot capacity beyond the in-LDS row sort");
+        return TCR_ERR_ARG;
+    }
     int64_t blocks = (b.nq_total + 3) / 4;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_mlm_sort_short, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets,
-                       (const uint64_t *)scratch, out);
+                       (const uint64_t *)scratch, src_cap, out);
     hipLaunchKernelGGL(k_mlm_sort_rows, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets, scratch,
-                       out);
+                       src_cap, out);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }

@@ -365,9 +365,17 @@ def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.in
     return plan.run(csr=csr)
 
 
+# Keys per query the multi-level count pass may keep (its single-traversal
+# mode), and the bound on the slot buffer that allows.
+MLM_SLOT_CAP = 128
+MLM_SLOT_BYTES_MAX = 16 << 30
+
+
 def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_levels: int = 4) -> "SearchResult":
     """Multi-level search of one cloud (same output as ``run_search([spec])``).
-    Raises ``_lib.Unsupported`` for inputs it does not take."""
+    Raises ``_lib.Unsupported`` for inputs it does not take.  The count pass
+    keeps each query's keys in a slot of MLM_SLOT_CAP; when every row fits,
+    the fill only sorts them (one traversal instead of two)."""
     if index_dtype not in (torch.int64, torch.int32):
         raise ValueError("index_dtype must be torch.int64 or torch.int32")
     L = _lib.load()
@@ -379,19 +387,27 @@ def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_level
         check(_lib.TCR_ERR_ARG)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
     offsets = torch.empty(nq + 1, dtype=torch.int64, device=device)
+    cap = MLM_SLOT_CAP if 0 < nq * MLM_SLOT_CAP * 8 <= MLM_SLOT_BYTES_MAX else 0
+    slots = torch.empty(nq * cap, dtype=torch.int64, device=device) if cap else None
     total = ctypes.c_int64(0)
     levels = ctypes.c_int32(0)
+    complete = ctypes.c_int32(0)
     stream = _stream()
     check(L.tcr_mlm_count(clouds, int(max_levels), ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
-                          ptr(offsets), ctypes.byref(total), ctypes.byref(levels), stream))
+                          ptr(offsets), ctypes.byref(total), ctypes.byref(levels), ptr(slots), cap,
+                          ctypes.byref(complete), stream))
     P = int(total.value)
     col = torch.empty(P, dtype=index_dtype, device=device)
     row = torch.empty(P, dtype=index_dtype, device=device) if want_rows else None
     if P > 0:
-        scratch = col if index_dtype == torch.int64 else torch.empty(P, dtype=torch.int64, device=device)
-        check(L.tcr_mlm_fill(clouds, levels.value, ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
-                             ptr(offsets), ptr(row), ptr(col), 8 if index_dtype == torch.int64 else 4, ptr(scratch),
-                             stream))
+        ib = 8 if index_dtype == torch.int64 else 4
+        if complete.value:
+            check(L.tcr_mlm_fill(clouds, levels.value, ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
+                                 ptr(offsets), ptr(row), ptr(col), ib, None, ptr(slots), cap, stream))
+        else:
+            scratch = col if index_dtype == torch.int64 else torch.empty(P, dtype=torch.int64, device=device)
+            check(L.tcr_mlm_fill(clouds, levels.value, ptr(qpos), ptr(qsup), ptr(rpos), ptr(rsup), ptr(ws), ws_bytes,
+                                 ptr(offsets), ptr(row), ptr(col), ib, ptr(scratch), None, 0, stream))
     return SearchResult(offsets, row, col, P, ws, clouds)
 
 
