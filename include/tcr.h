@@ -176,8 +176,9 @@ int tcr_search_finish(const tcr_cloud *clouds, int32_t n_clouds,
  * the single-level search then.  tcr_mlm_count synchronises twice and writes
  * offsets[nq + 1], the pair total and the level count to pass to
  * tcr_mlm_fill; scratch holds total_pairs uint64 and may be col itself when
- * index_bytes is 8.  With slots (uint64[nq * slot_cap], slot_cap <= 2048) the
- * count pass also writes every row's keys into its query's slot and sets
+ * index_bytes is 8.  With slots (uint32[nq * slot_cap], slot_cap <= 2048;
+ * fewer than 2^27 reference points) the count pass also writes every row's
+ * (stencil offset << 27 | index) keys into its query's slot and sets
  * *slots_complete when all rows fit; tcr_mlm_fill given those slots then
  * sorts the rows from there without a second traversal (scratch unused).
  * Otherwise pass slots = NULL to tcr_mlm_fill. */
@@ -186,12 +187,12 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels,
                   const float *qpos, const float *qsup, const float *rpos, const float *rsup,
                   void *workspace, size_t workspace_bytes, int64_t *offsets,
                   int64_t *total_pairs, int32_t *levels_used,
-                  uint64_t *slots, int64_t slot_cap, int32_t *slots_complete, void *stream);
+                  uint32_t *slots, int64_t slot_cap, int32_t *slots_complete, void *stream);
 int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels,
                  const float *qpos, const float *qsup, const float *rpos, const float *rsup,
                  void *workspace, size_t workspace_bytes, const int64_t *offsets,
                  void *row, void *col, int32_t index_bytes, uint64_t *scratch,
-                 const uint64_t *slots, int64_t slot_cap, void *stream);
+                 const uint32_t *slots, int64_t slot_cap, void *stream);
 
 /* Copy one cloud's structure out of the workspace in the reference layout:
  * sort_index int64[nr], sorted_pos fp32[nr, dim], sorted_sup fp32[nr] (NULL to

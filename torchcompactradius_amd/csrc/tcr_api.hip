@@ -561,7 +561,7 @@ size_t tcr_mlm_workspace_size(const tcr_cloud *cloud) {
 
 int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos, const float *qsup,
                   const float *rpos, const float *rsup, void *workspace, size_t workspace_bytes, int64_t *offsets,
-                  int64_t *total_pairs, int32_t *levels_used, uint64_t *slots, int64_t slot_cap,
+                  int64_t *total_pairs, int32_t *levels_used, uint32_t *slots, int64_t slot_cap,
                   int32_t *slots_complete, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (!cloud || !offsets) {
@@ -570,6 +570,10 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
     }
     if (slots && (slot_cap < 1 || slot_cap > kMlmSlotMax)) {
         set_error("slot_cap must be 1.." + std::to_string(kMlmSlotMax));
+        return TCR_ERR_ARG;
+    }
+    if (slots && cloud->nr >= ((int64_t)1 << 27)) {
+        set_error("32-bit slot keys hold reference indices below 2^27");
         return TCR_ERR_ARG;
     }
     if (max_levels < 1 || max_levels > 4) {
@@ -646,7 +650,7 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
 
 int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels, const float *qpos, const float *qsup, const float *rpos,
                  const float *rsup, void *workspace, size_t workspace_bytes, const int64_t *offsets, void *row,
-                 void *col, int32_t index_bytes, uint64_t *scratch, const uint64_t *slots, int64_t slot_cap,
+                 void *col, int32_t index_bytes, uint64_t *scratch, const uint32_t *slots, int64_t slot_cap,
                  void *stream) {
     hipStream_t s = (hipStream_t)stream;
     PairOut out;
@@ -667,7 +671,7 @@ int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels, const float *qpos, cons
     int rc = prepare(cloud, 1, qpos, qsup, rpos, rsup, workspace, workspace_bytes, s, p, false, true, 4);
     if (rc) return rc;
     if (slots)  // every row is in its slot already: sort from there
-        return mlm_sort_rows_launch(p.b, offsets, const_cast<uint64_t *>(slots), out, s, slot_cap);
+        return mlm_sort_rows_launch(p.b, offsets, nullptr, out, s, slots, slot_cap);
     rc = mlm_query_launch(p.b, p.w, levels, true, offsets, scratch, s);
     if (rc) return rc;
     return mlm_sort_rows_launch(p.b, offsets, scratch, out, s);

@@ -220,7 +220,7 @@ struct PairOut {
 // largest slot of the multi-level emit pass (rows sort in LDS up to this length)
 constexpr int64_t kMlmSlotMax = 2048;
 int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s,
-                         int64_t src_cap = 0);
+                         const uint32_t *slots = nullptr, int64_t slot_cap = 0);
 
 // Structure build (hMax, grid, keys, sort, cells, lin table).  Expects the
 // header block (descriptors, prefixes, zeroed misc) uploaded.
@@ -236,7 +236,7 @@ int cloud_cells_launch(const Batch &b, Work &w, hipStream_t s);
 int mlm_keys_launch(const Batch &b, Work &w, int levels, hipStream_t s);
 int mlm_groups_launch(const Batch &b, Work &w, hipStream_t s);
 int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64_t *offsets, uint64_t *scratch,
-                     hipStream_t s, uint64_t *slots = nullptr, int64_t slot_cap = 0);
+                     hipStream_t s, uint32_t *slots = nullptr, int64_t slot_cap = 0);
 // Count pass (counts[] and the fill's masks); fill pass into offsets,
 // skipped on the device when offsets[nq_total] > capacity.
 int query_count(const Batch &b, Work &w, hipStream_t s);

@@ -48,6 +48,7 @@ constexpr int kMlmBlock = 256;
 constexpr int kMlmLevelShift = 29;
 constexpr uint32_t kMlmGroupBit = 0x80000000u;
 constexpr int kMlmSortMax = 2048;           // rows up to this length sort in LDS (16 KiB per wavefront)
+constexpr int kMlmKey32Shift = 27;          // 32-bit slot keys: stencil offset above 27 index bits
 
 enum : uint32_t { kMlmBadSupport = 1, kMlmOutOfGrid = 2, kMlmQueryOutOfGrid = 4, kMlmSlotOverflow = 8 };
 
@@ -252,8 +253,8 @@ struct MlmArgs {
     int32_t *counts;
     const int64_t *offsets;
     uint64_t *scratch;   // fill: (stencil offset << 32 | original index) per hit
-    uint64_t *slots;     // emit: the same keys, query g's at [g * slot_cap, (g + 1) * slot_cap)
-    int64_t slot_cap;
+    uint32_t *slots;     // emit: (stencil offset << 27 | original index), query g's at
+    int64_t slot_cap;    // [g * slot_cap, (g + 1) * slot_cap); references < 2^27
 };
 
 // Pair support bound for a query of support hi against any point of a
@@ -311,7 +312,8 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
         if (!support_pred(dist, hi, v.w, mode)) return;
         const uint64_t key = ((uint64_t)(uint32_t)o << 32) | (uint32_t)a.sort_index[j];
         if (FILL) a.scratch[base + k] = key;
-        if (PASS == kMlmEmit && k < a.slot_cap) a.slots[g * a.slot_cap + k] = key;
+        if (PASS == kMlmEmit && k < a.slot_cap)
+            a.slots[g * a.slot_cap + k] = ((uint32_t)o << kMlmKey32Shift) | (uint32_t)a.sort_index[j];
         ++k;
     };
 
@@ -397,7 +399,7 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_query(MlmArgs a) {
 }
 
 int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64_t *offsets, uint64_t *scratch,
-                     hipStream_t s, uint64_t *slots, int64_t slot_cap) {
+                     hipStream_t s, uint32_t *slots, int64_t slot_cap) {
     if (b.nq_total == 0) return TCR_OK;
     MlmArgs a;
     a.desc = w.desc;
@@ -429,20 +431,35 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront per row: sort the row's (offset << 32 | index) keys and
-// write the original indices.  Two launches: SHORT takes the rows of at
-// most 64 keys and sorts them in registers (one key per lane, bitonic
-// network over shuffles; no LDS, so full occupancy); the other takes the
-// longer rows, up to kMlmSortMax in LDS (bitonic network over the next power
-// of two, padded with all-ones keys), and has lane 0 sort still longer rows
-// in place (insertion sort).  `scratch` may alias the int64 column output:
-// every entry is read before it is overwritten.  With src_cap > 0 the keys
-// come from the emit pass's slots (query g's at g * src_cap), not from the
-// row's own offset; such rows are at most src_cap <= kMlmSortMax long.
-__device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t src, int64_t m,
-                                               const uint64_t *scratch, const PairOut &out) {
+// One wavefront per row: sort the row's keys and write the original
+// indices.  Keys are (offset << 32 | index) (K = uint64_t: the second
+// traversal's keys at the row's own offset) or (offset << 27 | index)
+// (K = uint32_t: the emit pass's slots, query g's at g * src_cap; rows at
+// most src_cap <= kMlmSortMax long); both orders are (stencil offset,
+// original index).  Two launches: SHORT takes the rows of at most 64 keys and
+// sorts them in registers (one key per lane, bitonic network over shuffles;
+// no LDS, so full occupancy); the other takes the longer rows, up to
+// kMlmSortMax in LDS (bitonic network over the next power of two, padded
+// with all-ones keys), and has lane 0 sort still longer rows in place
+// (insertion sort; 64-bit keys only).  The 64-bit `scratch` may alias the
+// int64 column output: every entry is read before it is overwritten.
+template <typename K>
+__device__ __forceinline__ int32_t key_index(K key) {
+    return sizeof(K) == 8 ? (int32_t)(uint32_t)key : (int32_t)((uint32_t)key & ((1u << kMlmKey32Shift) - 1u));
+}
+template <typename K>
+__device__ __forceinline__ K shfl_xor_key(K key, int j) {
+    if (sizeof(K) == 4) return (K)(uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
+    const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
+    const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)((uint64_t)key >> 32), j, kWave);
+    return (K)(((uint64_t)ohi << 32) | olo);
+}
+
+template <typename K>
+__device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t src, int64_t m, const K *keys,
+                                               const PairOut &out) {
     const int lane = __lane_id();
-    uint64_t key = lane < m ? scratch[src + lane] : ~0ull;
+    K key = lane < m ? keys[src + lane] : (K)~(K)0;
     // a network over the next power of two >= m: lanes beyond it never pair
     // with lanes below it
 #pragma unroll
@@ -450,33 +467,32 @@ __device__ __forceinline__ void sort_row_short(int64_t g, int64_t b, int64_t src
         if ((kk >> 1) >= m) break;  // uniform: the first kk/2 lanes hold the whole row
 #pragma unroll
         for (int j = kk >> 1; j > 0; j >>= 1) {
-            const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
-            const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), j, kWave);
-            const uint64_t other = ((uint64_t)ohi << 32) | olo;
+            const K other = shfl_xor_key(key, j);
             // pair (i, i ^ j): the lower lane keeps the minimum in an
             // ascending block (lane & kk == 0), the maximum otherwise
             const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
             key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
         }
     }
-    if (lane < m) out.put(b + lane, g, (int32_t)(uint32_t)key);
+    if (lane < m) out.put(b + lane, g, key_index(key));
 }
 
+template <typename K>
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_short(int64_t nq, const int64_t *__restrict__ offsets,
-                                                              const uint64_t *scratch, int64_t src_cap,
-                                                              PairOut out) {
+                                                              const K *keys, int64_t src_cap, PairOut out) {
     const int64_t wid = ((int64_t)blockIdx.x * kMlmBlock + threadIdx.x) / kWave;
     const int64_t nw = ((int64_t)gridDim.x * kMlmBlock) / kWave;
     for (int64_t g = wid; g < nq; g += nw) {
         const int64_t b = offsets[g], m = offsets[g + 1] - b;
-        if (m > 0 && m <= kWave) sort_row_short(g, b, src_cap > 0 ? g * src_cap : b, m, scratch, out);
+        if (m > 0 && m <= kWave) sort_row_short<K>(g, b, src_cap > 0 ? g * src_cap : b, m, keys, out);
     }
 }
 
+template <typename K>
 __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const int64_t *__restrict__ offsets,
-                                                             uint64_t *scratch, int64_t src_cap, PairOut out) {
-    __shared__ uint64_t s_keys[kMlmBlock / kWave][kMlmSortMax];
-    uint64_t *sk = s_keys[threadIdx.x / kWave];
+                                                             K *keys, int64_t src_cap, PairOut out) {
+    __shared__ K s_keys[kMlmBlock / kWave][kMlmSortMax];
+    K *sk = s_keys[threadIdx.x / kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wid = ((int64_t)blockIdx.x * kMlmBlock + threadIdx.x) / kWave;
     const int64_t nw = ((int64_t)gridDim.x * kMlmBlock) / kWave;
@@ -487,13 +503,13 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const i
         if (m <= kMlmSortMax) {
             int n2 = 1;
             while (n2 < m) n2 <<= 1;
-            for (int t = lane; t < n2; t += kWave) sk[t] = t < m ? scratch[src + t] : ~0ull;
+            for (int t = lane; t < n2; t += kWave) sk[t] = t < m ? keys[src + t] : (K)~(K)0;
             wave_lds_sync();
             for (int kk = 2; kk <= n2; kk <<= 1)
                 for (int j = kk >> 1; j > 0; j >>= 1) {
                     for (int t = lane; t < n2 / 2; t += kWave) {
                         const int i = ((t / j) * 2 * j) + (t % j), p = i + j;
-                        const uint64_t x = sk[i], y = sk[p];
+                        const K x = sk[i], y = sk[p];
                         const bool asc = (i & kk) == 0;
                         if ((x > y) == asc) {
                             sk[i] = y;
@@ -502,36 +518,46 @@ __global__ void __launch_bounds__(kMlmBlock) k_mlm_sort_rows(int64_t nq, const i
                     }
                     wave_lds_sync();
                 }
-            for (int t = lane; t < m; t += kWave) out.put(b + t, g, (int32_t)(uint32_t)sk[t]);
+            for (int t = lane; t < m; t += kWave) out.put(b + t, g, key_index(sk[t]));
             wave_lds_sync();
-        } else if (lane == 0) {
+        } else if (sizeof(K) == 8 && lane == 0) {
             for (int64_t i = 1; i < m; ++i) {
-                const uint64_t x = scratch[b + i];
+                const K x = keys[b + i];
                 int64_t j = i - 1;
-                while (j >= 0 && scratch[b + j] > x) {
-                    scratch[b + j + 1] = scratch[b + j];
+                while (j >= 0 && keys[b + j] > x) {
+                    keys[b + j + 1] = keys[b + j];
                     --j;
                 }
-                scratch[b + j + 1] = x;
+                keys[b + j + 1] = x;
             }
-            for (int64_t t = 0; t < m; ++t) out.put(b + t, g, (int32_t)(uint32_t)scratch[b + t]);
+            for (int64_t t = 0; t < m; ++t) out.put(b + t, g, key_index(keys[b + t]));
         }
     }
 }
 
-int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s,
-                         int64_t src_cap) {
-    if (b.nq_total == 0) return TCR_OK;
-    if (src_cap > kMlmSortMax) {
-        set_error("slot capacity beyond the in-LDS row sort");
-        return TCR_ERR_ARG;
-    }
+template <typename K>
+static void sort_launch(const Batch &b, const int64_t *offsets, K *keys, int64_t src_cap, PairOut out,
+                        hipStream_t s) {
     int64_t blocks = (b.nq_total + 3) / 4;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_mlm_sort_short, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets,
-                       (const uint64_t *)scratch, src_cap, out);
-    hipLaunchKernelGGL(k_mlm_sort_rows, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets, scratch,
+    hipLaunchKernelGGL(k_mlm_sort_short<K>, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets,
+                       (const K *)keys, src_cap, out);
+    hipLaunchKernelGGL(k_mlm_sort_rows<K>, dim3((unsigned)blocks), dim3(kMlmBlock), 0, s, b.nq_total, offsets, keys,
                        src_cap, out);
+}
+
+int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scratch, PairOut out, hipStream_t s,
+                         const uint32_t *slots, int64_t slot_cap) {
+    if (b.nq_total == 0) return TCR_OK;
+    if (slots) {
+        if (slot_cap < 1 || slot_cap > kMlmSortMax) {
+            set_error("slot capacity beyond the in-LDS row sort");
+            return TCR_ERR_ARG;
+        }
+        sort_launch<uint32_t>(b, offsets, const_cast<uint32_t *>(slots), slot_cap, out, s);
+    } else {
+        sort_launch<uint64_t>(b, offsets, scratch, 0, out, s);
+    }
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }

@@ -387,8 +387,9 @@ def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_level
         check(_lib.TCR_ERR_ARG)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
     offsets = torch.empty(nq + 1, dtype=torch.int64, device=device)
-    cap = MLM_SLOT_CAP if 0 < nq * MLM_SLOT_CAP * 8 <= MLM_SLOT_BYTES_MAX else 0
-    slots = torch.empty(nq * cap, dtype=torch.int64, device=device) if cap else None
+    # 32-bit slot keys: (stencil offset << 27 | reference index)
+    cap = MLM_SLOT_CAP if 0 < nq * MLM_SLOT_CAP * 4 <= MLM_SLOT_BYTES_MAX and nr < (1 << 27) else 0
+    slots = torch.empty(nq * cap, dtype=torch.int32, device=device) if cap else None
     total = ctypes.c_int64(0)
     levels = ctypes.c_int32(0)
     complete = ctypes.c_int32(0)
