@@ -87,6 +87,41 @@ def test_radius_with_batch_ids():
     assert got == want
 
 
+@pytest.mark.parametrize("algorithm", ["naive", "small", "mlm"])
+def test_radius_algorithms(algorithm):
+    """radius() with radiusSearch's other algorithms, with and without batch
+    ids: the flipped pairs of each algorithm's own search (for 'naive', its
+    '<=' and default-domain semantics)."""
+    rng = np.random.default_rng(78 + len(algorithm))
+    n, m = 900, 700
+    x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    y = rng.uniform(0, 1, (m, 2)).astype(np.float32)
+    hx = rng.uniform(0.04, 0.07, n).astype(np.float32)
+    hy = rng.uniform(0.04, 0.07, m).astype(np.float32)
+    bx = np.sort(rng.integers(0, 3, n))
+    by = np.sort(rng.integers(0, 3, m))
+    mode = "symmetric"
+    print(f"case: radius algorithm={algorithm}", flush=True)
+
+    def want_pairs(qi, ri):
+        if algorithm == "naive":
+            ii, jj = oracle.radius_search_naive(x[qi], hx[qi], y[ri], hy[ri], mode=mode)
+        elif algorithm == "small":
+            ii, jj = oracle.radius_search_small(x[qi], hx[qi], y[ri], hy[ri], mode=mode)
+        else:
+            o = oracle.radius_search(x[qi], hx[qi], y[ri], hy[ri], mode=mode)
+            ii, jj = o.row, o.col
+        return set(zip(ri[jj].tolist(), qi[ii].tolist()))
+
+    i, j = radius(_t(x), _t(y), (_t(hx), _t(hy)), mode=mode, algorithm=algorithm)
+    assert set(zip(i.cpu().tolist(), j.cpu().tolist())) == want_pairs(np.arange(n), np.arange(m))
+    i, j = radius(_t(x), _t(y), (_t(hx), _t(hy)), batch_x=_t(bx), batch_y=_t(by), mode=mode, algorithm=algorithm)
+    want = set()
+    for b in range(3):
+        want |= want_pairs(np.nonzero(bx == b)[0], np.nonzero(by == b)[0])
+    assert set(zip(i.cpu().tolist(), j.cpu().tolist())) == want
+
+
 @pytest.mark.parametrize("mode", ["gather", "scatter", "symmetric", "superSymmetric"])
 def test_native_ops_on_exported_structure(mode):
     """searchNeighbors_cpp's sequence (compactHashing/cppSearch.py:60-98) on

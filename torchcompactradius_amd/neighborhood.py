@@ -255,8 +255,12 @@ def radius(queryPositions: torch.Tensor,
     """torch_cluster-style radius (neighborhood.py:313-365, README.md:45):
     returns ``(i, j)`` with ``i`` indexing the reference set and ``j`` the
     query set (flipped relative to radiusSearch).  With ``batch_x`` /
-    ``batch_y``, each batch id is searched separately -- all ids in one
-    batched launch -- and indices are offset into the concatenated sets."""
+    ``batch_y``, each batch id is searched separately and indices are offset
+    into the concatenated sets: for 'compact' all ids in one batched launch,
+    for the other algorithms one radiusSearch per id.  ``algorithm`` is
+    radiusSearch's (the reference's own radius passes its arguments to
+    radiusSearch out of order and cannot run, so its default 'naive' defines
+    no result; 'compact' is the default here)."""
     with torch.no_grad():
         if isinstance(support, tuple):
             qsup, rsup = support
@@ -271,7 +275,7 @@ def radius(queryPositions: torch.Tensor,
         if batch_x is None and batch_y is None:
             ref = None if referencePositions is None else PointCloud(referencePositions, rsup)
             res = radiusSearch(PointCloud(queryPositions, qsup), ref, mode=mode, domain=domain,
-                               hashMapLength=hashMapLength, algorithm="compact", format="coo",
+                               hashMapLength=hashMapLength, algorithm=algorithm, format="coo",
                                returnStructure=returnStructure)
             coo = res[0] if returnStructure else res
             if returnStructure:
@@ -284,6 +288,26 @@ def radius(queryPositions: torch.Tensor,
         by = batch_y if batch_y is not None else (bx if referencePositions is None else torch.zeros(
             ref_pos.shape[0], dtype=torch.long, device=ref_pos.device))
         ids = torch.unique(torch.cat([bx, by])).tolist()
+        if algorithm != "compact":
+            assert algorithm in ALGORITHMS, f"algorithm = {algorithm} not supported"
+            cols, rows = [], []
+            for b in ids:
+                qi = torch.nonzero(bx == b).reshape(-1)
+                ri = torch.nonzero(by == b).reshape(-1)
+                out = radiusSearch(PointCloud(queryPositions[qi].contiguous(),
+                                              None if qsup is None else qsup[qi].contiguous()),
+                                   PointCloud(ref_pos[ri].contiguous(),
+                                              None if ref_sup is None else ref_sup[ri].contiguous()),
+                                   mode=mode, domain=domain, hashMapLength=hashMapLength, algorithm=algorithm,
+                                   format="coo")
+                rows.append(qi[out.row])
+                cols.append(ri[out.col])
+            empty = torch.empty(0, dtype=torch.long, device=queryPositions.device)
+            cols_global = torch.cat(cols) if cols else empty
+            rows_global = torch.cat(rows) if rows else empty
+            if returnStructure:
+                return cols_global, rows_global, None
+            return cols_global, rows_global
         specs, qidx, ridx = [], [], []
         for b in ids:
             qi = torch.nonzero(bx == b).reshape(-1)
