@@ -178,3 +178,32 @@ def test_small_known_answers_and_fixed(periodic):
                                             periodic=[periodic] * 2, mode="gather", support_override=override)
         np.testing.assert_array_equal(full.row.cpu().numpy(), ii)
         np.testing.assert_array_equal(full.col.cpu().numpy(), jj)
+
+
+@pytest.mark.parametrize("algorithm", ["naive", "small"])
+def test_non_finite_inputs(algorithm):
+    """NaN / inf positions and supports, zero and negative supports: every
+    comparison with NaN fails, inf distances exceed every finite support, and
+    the kernels agree with the restatements."""
+    rng = np.random.default_rng(55)
+    n = 600
+    x = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    h = rng.uniform(0.05, 0.15, n).astype(np.float32)
+    x[3, 0] = np.nan
+    x[7, 1] = np.inf
+    x[11, 2] = -np.inf
+    h[5] = np.nan
+    h[9] = np.inf
+    h[13] = 0.0
+    h[17] = -0.1
+    mn, mx = np.zeros(3, np.float32), np.ones(3, np.float32)
+    per = [True, False, True]
+    print(f"case: {algorithm} non-finite inputs", flush=True)
+    for mode in ("gather", "scatter", "symmetric", "superSymmetric"):
+        out = radiusSearch(PointCloud(_t(x), _t(h)), None, mode=mode, domain=_dom(mn, mx, per), algorithm=algorithm)
+        if algorithm == "naive":
+            ii, jj = oracle.radius_search_naive(x, h, None, None, domain_min=mn, domain_max=mx, periodic=per, mode=mode)
+        else:
+            ii, jj = oracle.radius_search_small(x, h, None, None, domain_min=mn, domain_max=mx, periodic=per, mode=mode)
+        np.testing.assert_array_equal(out.row.cpu().numpy(), ii)
+        np.testing.assert_array_equal(out.col.cpu().numpy(), jj)
