@@ -141,7 +141,6 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
     w.masks = cv.take<uint64_t>(7 * (size_t)nr);
     w.ranges = cv.take<int2>(27 * (size_t)nr);
     w.recs = cv.take<float4>(2 * (size_t)nr);
-    w.soff = cv.take<int64_t>(nr);
     w.big_list = cv.take<uint32_t>(nr);
     return cv.off + 256;
 }
@@ -275,7 +274,10 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     if (rc) return rc;
     rc = query_count(p.b, p.w, s);
     if (rc) return rc;
-    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
+    // the scan writes the CSR arrays too (before the fill, which the
+    // device skips when the total exceeds the capacity)
+    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s, csr.indptr,
+                                          csr.row_entries);
     if (rc) return rc;
     if (capacity >= 0) {
         if (!out.col) {
@@ -293,7 +295,7 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
             ke.before = pe.e[1];
             ke.after = pe.e[3];
         }
-        rc = query_fill(p.b, p.w, offsets, out, capacity, s, csr);
+        rc = query_fill(p.b, p.w, offsets, out, capacity, s);
         if (timed && !ke.recorded) {  // no fill kernel ran
             if (pe.e[1]) hipEventRecord(pe.e[1], s);
             if (pe.e[3]) hipEventRecord(pe.e[3], s);

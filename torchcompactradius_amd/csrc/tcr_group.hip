@@ -151,7 +151,6 @@ struct GroupArgs {
     const int64_t *offsets;      // [nq+1] (fill)
     const int64_t *total;        // offsets + nq: the fill runs only if *total <= capacity
     int64_t capacity;
-    int64_t *soff;               // [nr] row offset of each sorted slot's query (fill)
     PairOut out;                 // fill outputs
     int32_t *cursor;             // [nq] per-query write cursor across chunks (fill)
     int32_t *stray_list;         // [nq]
@@ -645,8 +644,8 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
 #pragma unroll
     for (int r = 0; r < kSmallRounds; ++r)
         mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms0] : 0ull;
-    int64_t mbase = live ? a.soff[ms0] : 0;
     int32_t mloc = a.sort_index[ms0];
+    int64_t mbase = live ? a.offsets[v.qs + mloc] : 0;
 
     expand_slots(v, slot_off, noff);
     int32_t corig[kSmallRounds];
@@ -664,8 +663,8 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
                 mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms] : 0ull;
-            mbase = live ? a.soff[ms] : 0;
             mloc = a.sort_index[ms];
+            mbase = live ? a.offsets[v.qs + mloc] : 0;
         }
         const uint64_t todo = __ballot(live);
         if (!todo) continue;
@@ -847,27 +846,6 @@ __global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a) {
     }
 }
 
-// Row offset of each sorted slot's query (self-search clouds): the fill's
-// per-query base without a dependent gather inside the kernel.  When asked
-// for, the same launch writes the CSR arrays of the offsets (indptr int32
-// [nq+1], rowEntries int64 [nq]).
-__global__ void __launch_bounds__(256) k_sorted_offsets(const CloudDev *__restrict__ desc, int32_t B,
-                                                        const int64_t *__restrict__ r_start,
-                                                        const int64_t *__restrict__ q_start, int64_t n,
-                                                        const int32_t *__restrict__ sort_index,
-                                                        const int64_t *__restrict__ offsets,
-                                                        int64_t *__restrict__ soff, int64_t nq,
-                                                        int32_t *__restrict__ indptr,
-                                                        int64_t *__restrict__ row_entries) {
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s < n) {
-        const int c = find_segment(B, s, [&](int k) { return r_start[k]; });
-        if (group_owned(desc[c])) soff[s] = offsets[q_start[c] + sort_index[s]];
-    }
-    if (indptr && s <= nq) indptr[s] = (int32_t)offsets[s];
-    if (row_entries && s < nq) row_entries[s] = offsets[s + 1] - offsets[s];
-}
-
 // ------------------------------------------------------------- driver -----
 
 // Workgroups of `kernel` that the current device holds at once (all CUs),
@@ -964,7 +942,6 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
     ga.offsets = offsets;
     ga.total = offsets ? offsets + b.nq_total : nullptr;
     ga.capacity = capacity;
-    ga.soff = w.soff;
     ga.out = out;
     ga.cursor = w.cursor;
     ga.stray_list = w.stray_list;
@@ -978,18 +955,12 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
 // self-search clouds (their strays collected by the count pass), the
 // per-query kernel for every other cloud and for the strays.
 static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
-                      hipStream_t s, CsrOut csr = CsrOut{}) {
+                      hipStream_t s) {
     bool any_group = false, any_other = false;
     for (const CloudDev &d : b.dev) {
         if (d.h.nq == 0) continue;
         if (d.h.self_search && d.h.search_range == 1) any_group = true;
         else any_other = true;
-    }
-    // the CSR arrays, if asked for, are written before the fill kernels:
-    // by k_sorted_offsets when it runs, else by their own launch
-    if (fill && (csr.indptr || csr.row_entries) && !any_group) {
-        const int rc = csr_arrays_launch(offsets, b.nq_total, csr.indptr, csr.row_entries, s);
-        if (rc) return rc;
     }
     if (b.nq_total == 0) return TCR_OK;
     if (any_group) {
@@ -1002,12 +973,6 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
             const int32_t sw = tuning().stencil_workgroups;
             const unsigned sgrid = sw > 0 ? (unsigned)sw : (unsigned)std::max<int64_t>(1, (b.nr_total + 255) / 256);
             hipLaunchKernelGGL(k_stencil, dim3(sgrid), dim3(256), 0, s, ga);
-        } else {
-            const int64_t span = std::max<int64_t>(b.nr_total, csr.indptr || csr.row_entries ? b.nq_total + 1 : 0);
-            hipLaunchKernelGGL(k_sorted_offsets, dim3((unsigned)((span + 255) / 256)), dim3(256), 0, s,
-                               (const CloudDev *)w.desc, b.B, (const int64_t *)w.r_start,
-                               (const int64_t *)w.q_start, b.nr_total, (const int32_t *)w.sort_index, offsets,
-                               w.soff, b.nq_total, csr.indptr, csr.row_entries);
         }
         int rc = fill ? launch_modes<true>(b, ga, s) : launch_modes<false>(b, ga, s);
         if (rc) return rc;
@@ -1021,9 +986,8 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
 int query_count(const Batch &b, Work &w, hipStream_t s) {
     return query_pass(b, w, nullptr, PairOut{nullptr, nullptr, 0}, -1, false, s);
 }
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s,
-               CsrOut csr) {
-    return query_pass(b, w, offsets, out, capacity, true, s, csr);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s) {
+    return query_pass(b, w, offsets, out, capacity, true, s);
 }
 
 }  // namespace tcr

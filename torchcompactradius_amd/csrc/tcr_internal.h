@@ -117,7 +117,6 @@ struct Work {
     uint64_t *masks;      // [Nr * 7] per sorted slot: hit mask of each candidate round
     int2 *ranges;         // [Nr * 27] per cell: stencil cell (first global slot, count)
     float4 *recs;         // [Nr * 2] per cell: 32-byte record (slots, cloud, state, centre)
-    int64_t *soff;        // [Nr] row offset of each sorted slot's query
     uint32_t *big_list;   // [Nr] cells with more candidates than one register chunk
 };
 
@@ -187,10 +186,11 @@ struct ClearSpans {
 
 // Generic device-wide exclusive scan: out[i] = sum(in[0..i)), and
 // out[n] = total when write_total.  Tin: uint32_t or int32_t;
-// Tout: uint32_t or int64_t.
+// Tout: uint32_t or int64_t.  The same launches write the CSR arrays of
+// int64 offsets when asked for: indptr int32 [n+1], rowEntries int64 [n].
 template <typename Tin, typename Tout>
 int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
-                   hipStream_t s);
+                   hipStream_t s, int32_t *indptr = nullptr, int64_t *row_entries = nullptr);
 size_t scan_partials_needed(int64_t n);
 int csr_arrays_launch(const int64_t *offsets, int64_t n, int32_t *indptr, int64_t *row_entries, hipStream_t s);
 
@@ -240,13 +240,12 @@ int mlm_query_launch(const Batch &b, Work &w, int levels, bool fill, const int64
 // Count pass (counts[] and the fill's masks); fill pass into offsets,
 // skipped on the device when offsets[nq_total] > capacity.
 int query_count(const Batch &b, Work &w, hipStream_t s);
+int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s);
 // CSR arrays of a search (either may be null): indptr int32 [nq+1], rowEntries int64 [nq].
 struct CsrOut {
     int32_t *indptr = nullptr;
     int64_t *row_entries = nullptr;
 };
-int query_fill(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, hipStream_t s,
-               CsrOut csr = CsrOut{});
 // Reference-layout bucket tables of every cloud (for export).
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted

@@ -91,10 +91,15 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_partials(uint64_t *__rest
     if (threadIdx.x == 0) part[nb] = carry;
 }
 
+// Applies the block prefixes; with int64 offsets it also writes the CSR
+// arrays when asked for (indptr int32 [n+1] = the offsets, rowEntries int64
+// [n] = the counts).
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const Tin *__restrict__ in, Tout *__restrict__ out,
                                                               int64_t n, const uint64_t *__restrict__ part,
-                                                              int64_t nb, int write_total) {
+                                                              int64_t nb, int write_total,
+                                                              int32_t *__restrict__ indptr,
+                                                              int64_t *__restrict__ row_entries) {
     __shared__ uint64_t lds[kScanTile];
     __shared__ uint64_t wt[kScanThreads / kWave];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
@@ -107,35 +112,47 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const Tin *__restri
     const uint64_t pre = part[blockIdx.x];
     for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
         const int64_t i = base + k;
-        if (i < n) out[i] = (Tout)(lds[k] + pre);
+        if (i < n) {
+            out[i] = (Tout)(lds[k] + pre);
+            if (indptr) indptr[i] = (int32_t)(int64_t)(lds[k] + pre);
+            if (row_entries) row_entries[i] = (int64_t)in[i];
+        }
     }
-    if (write_total && blockIdx.x == 0 && threadIdx.x == 0) out[n] = (Tout)part[nb];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (write_total) out[n] = (Tout)part[nb];
+        if (indptr) indptr[n] = (int32_t)(int64_t)part[nb];
+    }
 }
 
-__global__ void k_write_zero_total(uint64_t *part) { part[0] = 0; }
+// indptr of an empty query set: [0]
+__global__ void k_zero_indptr(int32_t *indptr) { indptr[0] = 0; }
 
 size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 2; }
 
 template <typename Tin, typename Tout>
 int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
-                   hipStream_t s) {
+                   hipStream_t s, int32_t *indptr, int64_t *row_entries) {
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 0) {
         if (write_total) TCR_HIP(hipMemsetAsync(out, 0, sizeof(Tout), s));
+        if (indptr) {
+            hipLaunchKernelGGL(k_zero_indptr, dim3(1), dim3(1), 0, s, indptr);
+            TCR_LAUNCH_CHECK();
+        }
         return TCR_OK;
     }
     hipLaunchKernelGGL(k_scan_reduce<Tin>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
     hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
-                       (const uint64_t *)partials, nb, write_total ? 1 : 0);
+                       (const uint64_t *)partials, nb, write_total ? 1 : 0, indptr, row_entries);
     TCR_LAUNCH_CHECK();
     return TCR_OK;
 }
 
 template int exclusive_scan<uint32_t, uint32_t>(const uint32_t *, uint32_t *, int64_t, bool, uint64_t *,
-                                                hipStream_t);
+                                                hipStream_t, int32_t *, int64_t *);
 template int exclusive_scan<int32_t, int64_t>(const int32_t *, int64_t *, int64_t, bool, uint64_t *,
-                                              hipStream_t);
+                                              hipStream_t, int32_t *, int64_t *);
 
 // CSR arrays of the reference's SparseCSR from the int64 offsets
 // (neighborhood.py:303-305, util.py:335-358): indptr int32[n+1] and
