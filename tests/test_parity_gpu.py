@@ -112,6 +112,40 @@ def test_domain_none_and_points_outside():
         assert_same(g, o)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("mode", ["gather", "symmetric"])
+def test_non_finite_positions(dim, periodic, mode):
+    """NaN and +-inf coordinates (no pairs: every distance to them is NaN,
+    their cells are INT32_MIN and alias through the hash) and a huge finite
+    one (out of the grid, or wrapped back into a periodic domain), in a
+    self-search and with a separate query set."""
+    rng = np.random.default_rng(30 + dim)
+    n = 2000
+    q = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    q[5, 0] = np.nan
+    q[17, dim - 1] = np.inf
+    q[40, 0] = -np.inf
+    q[41, :] = np.nan
+    q[99, 1] = np.float32(1e30)
+    h = np.full(n, 0.12, np.float32)
+    mn, mx = np.full(dim, -1, np.float32), np.full(dim, 1, np.float32)
+    per = [periodic] * dim
+    print(f"case: non-finite dim={dim} periodic={periodic} mode={mode}", flush=True)
+    for fmt in ("coo", "csr"):
+        g = gpu_search(q, h, mn=mn, mx=mx, periodic=per, mode=mode, fmt=fmt)
+        o = oracle.radius_search(q, h, domain_min=mn, domain_max=mx, periodic=per, mode=mode, hash_map_length=4096)
+        assert_same(g, o, fmt)
+    r = rng.uniform(-1, 1, (1500, dim)).astype(np.float32)
+    r[3, 0] = np.nan
+    r[8, dim - 1] = -np.inf
+    hr = np.full(1500, 0.12, np.float32)
+    g = gpu_search(q, h, r, hr, mn=mn, mx=mx, periodic=per, mode=mode)
+    o = oracle.radius_search(q, h, r, hr, domain_min=mn, domain_max=mx, periodic=per, mode=mode,
+                             hash_map_length=4096)
+    assert_same(g, o)
+
+
 def test_wrap_boundary_values():
     """Values that once-wrap onto max and twice-wrap onto min (the reference
     wraps the reference set twice and the query set once)."""
