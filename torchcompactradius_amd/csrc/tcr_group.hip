@@ -39,7 +39,7 @@
 #include <map>
 #include <mutex>
 
-#include "tcr_internal.h"
+#include "tcr_generic.h"
 
 namespace tcr {
 
@@ -833,8 +833,12 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     }
 }
 
-template <int DIM, int MODE, bool FILL>
-__global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a) {
+// Cells too big for the small-cell kernels, one wavefront per cell; with
+// LIST (the last launch of a pass) the wavefronts then take the stray
+// queries the count pass listed, one wavefront per query -- in the same
+// launch, since both lists are usually short or empty.
+template <int DIM, int MODE, bool FILL, bool LIST>
+__global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a, GenericArgs ga) {
     if (FILL && *a.total > a.capacity) return;
     const uint32_t n = *a.big_count;
     const uint32_t wpb = kGroupThreads / kWave;
@@ -843,6 +847,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a) {
         const CloudDev &cd = a.desc[uniform(a.cell_cloud[cid])];
         if (!cloud_is(cd, DIM, MODE)) continue;
         group_big<DIM, MODE, FILL>(a, cid);
+    }
+    if (LIST) {
+        const int64_t ns = *ga.list_count;
+        for (int64_t i = (int64_t)blockIdx.x * wpb + threadIdx.x / kWave; i < ns; i += (int64_t)gridDim.x * wpb)
+            generic_query_wave<FILL>(ga, ga.list[i]);
     }
 }
 
@@ -886,17 +895,26 @@ static unsigned group_grid(const Batch &b, const void *kernel) {
 }
 
 template <bool FILL>
-static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
+static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &gen, hipStream_t s) {
     bool seen[4][4] = {};
+    int last = -1;  // (dim, mode) of the last big-cell launch, which also takes the stray list
     for (const CloudDev &d : b.dev)
         if (d.h.self_search && d.h.search_range == 1 && d.h.nq > 0) seen[d.h.dim][d.h.mode] = true;
-    const dim3 block(kGroupThreads), bgrid(256);
+    for (int d = 1; d <= 3; ++d)
+        for (int m = 0; m < 4; ++m)
+            if (seen[d][m]) last = d * 4 + m;
+    // 1024 workgroups: the stray list's wavefront-per-query pass wants more
+    // waves than the big cells
+    const dim3 block(kGroupThreads), bgrid(256), lgrid(1024);
 #define TCR_L(D, M)                                                                                         \
     if (seen[D][M]) {                                                                                       \
         if (!FILL)                                                                                          \
             hipLaunchKernelGGL((k_count_small<D, M>), dim3(group_grid(b, (const void *)k_count_small<D, M>)), \
                                block, 0, s, ga);                                                            \
-        hipLaunchKernelGGL((k_group_big<D, M, FILL>), bgrid, block, 0, s, ga);                              \
+        if (last == D * 4 + M)                                                                              \
+            hipLaunchKernelGGL((k_group_big<D, M, FILL, true>), lgrid, block, 0, s, ga, gen);               \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_group_big<D, M, FILL, false>), bgrid, block, 0, s, ga, gen);              \
     }
 #define TCR_LD(D) TCR_L(D, 0) TCR_L(D, 1) TCR_L(D, 2) TCR_L(D, 3)
     if (!FILL) {
@@ -919,7 +937,7 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, hipStream_t s) {
 }
 
 int launch_generic(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
-                   bool list, hipStream_t s);
+                   hipStream_t s);
 
 static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity) {
     GroupArgs ga;
@@ -974,12 +992,11 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
             const unsigned sgrid = sw > 0 ? (unsigned)sw : (unsigned)std::max<int64_t>(1, (b.nr_total + 255) / 256);
             hipLaunchKernelGGL(k_stencil, dim3(sgrid), dim3(256), 0, s, ga);
         }
-        int rc = fill ? launch_modes<true>(b, ga, s) : launch_modes<false>(b, ga, s);
-        if (rc) return rc;
-        rc = launch_generic(b, w, offsets, out, capacity, fill, true, s);
+        const GenericArgs gen = generic_args(b, w, offsets, out, capacity);
+        const int rc = fill ? launch_modes<true>(b, ga, gen, s) : launch_modes<false>(b, ga, gen, s);
         if (rc) return rc;
     }
-    if (any_other) return launch_generic(b, w, offsets, out, capacity, fill, false, s);
+    if (any_other) return launch_generic(b, w, offsets, out, capacity, fill, s);
     return TCR_OK;
 }
 
