@@ -212,21 +212,36 @@ __device__ __forceinline__ void ref_key_one(const CloudDev &cd, int64_t local, f
     flags = f;
 }
 
-__global__ void __launch_bounds__(kBlock) k_keys(CloudDev *__restrict__ desc, int32_t B,
-                                                 const int64_t *__restrict__ r_start, int64_t n,
-                                                 float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
-                                                 uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    int c = -1;
+// One sort tile of reference points per workgroup: per point the second wrap,
+// cell key and stray bits (ref_key_one), and the tile's digit counts of
+// every radix pass, folded into the cloud's histograms -- the work of
+// k_sort_hist without its launch and its second read of the keys.
+// Histograms count raw key digits (a wide cloud's are mapped by
+// hist_digit); a tile that sees a point outside the grid marks the cloud
+// irregular and asks for every pass.
+__global__ void __launch_bounds__(kSortThreads) k_keys(CloudDev *__restrict__ desc, int32_t B,
+                                                       const int64_t *__restrict__ t_start,
+                                                       const int64_t *__restrict__ p_start,
+                                                       float4 *__restrict__ rwrap, uint32_t *__restrict__ keys,
+                                                       uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags,
+                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ misc) {
+    __shared__ uint32_t h[kMaxPasses][kRadix];
+    __shared__ uint32_t s_out;
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) (&h[0][0])[k] = 0;
+    if (threadIdx.x == 0) s_out = 0;
+    __syncthreads();
+    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
+    CloudDev &cd = desc[ti.c];
+    const int64_t cstart = p_start[ti.c];
+    const int dim = cd.h.dim;
     bool out = false;
-    if (g < n) {
-        c = cloud_of(B, g, r_start);
-        const CloudDev &cd = desc[c];
-        const int64_t local = g - r_start[c];
+    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
+        const int64_t g = ti.start + k;
+        const int64_t local = g - cstart;
         float4 wp;
         int32_t lin;
         uint8_t f;
-        switch (cd.h.dim) {
+        switch (dim) {
             case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
             case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
             default: ref_key_one<3>(cd, local, wp, lin, f); break;
@@ -235,14 +250,21 @@ __global__ void __launch_bounds__(kBlock) k_keys(CloudDev *__restrict__ desc, in
         keys[g] = (uint32_t)lin;
         vals[g] = (uint32_t)local;
         pflags[g] = f;
-        out = (f & 2) != 0;
+        out = out || (f & 2) != 0;
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p)
+            atomicAdd(&h[p][((uint32_t)lin >> (p * kRadixBits)) & (kRadix - 1)], 1u);
     }
-    // the cloud is irregular if any point's cell is outside the grid
-    const int c0 = __shfl(c, 0, kWave);
-    if (__all(c == c0)) {
-        if (__any(out) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(&desc[c0].irregular, 1u);
-    } else if (out) {
-        atomicOr(&desc[c].irregular, 1u);
+    if (out) s_out = 1u;
+    __syncthreads();
+    uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) {
+        const uint32_t v = (&h[0][0])[k];
+        if (v) atomicAdd(&hc[k], v);
+    }
+    if (threadIdx.x == 0) {
+        if (s_out) atomicOr(&cd.irregular, 1u);
+        atomicMax(&misc[kMiscPasses], s_out ? (uint32_t)kMaxPasses : cloud_passes(cd));
     }
 }
 
@@ -406,12 +428,13 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
     return TCR_OK;
 }
 
-int build_sorted_cells(const Batch &b, Work &w, hipStream_t s) {
+int build_sorted_cells(const Batch &b, Work &w, hipStream_t s, bool hist_done) {
     const int64_t n = b.nr_total;
     if (n == 0) return TCR_OK;
-    // the sort's histogram kernel also empties the cell index; build_prep
-    // zeroed the look-back state of the sort and of k_cells
-    int rc = radix_sort(b, w, s);
+    // the sort's histogram kernel (or its first pass, after k_keys) also
+    // empties the cell index; build_prep zeroed the look-back state of the
+    // sort and of k_cells
+    int rc = radix_sort(b, w, s, hist_done);
     if (rc) return rc;
     CellsArgs ca;
     ca.desc = w.desc;
@@ -447,10 +470,11 @@ int build_structure(const Batch &b, Work &w, hipStream_t s) {
     if (rc) return rc;
     const int64_t n = b.nr_total;
     if (n == 0) return TCR_OK;
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, w.desc, b.B, (const int64_t *)w.r_start, n,
-                       w.rwrap, w.keys0, w.vals0, w.pflags);
+    hipLaunchKernelGGL(k_keys, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, w.desc, b.B,
+                       (const int64_t *)w.rt_start, (const int64_t *)w.r_start, w.rwrap, w.keys0, w.vals0,
+                       w.pflags, w.sort_hist, w.misc);
     TCR_LAUNCH_CHECK();
-    return build_sorted_cells(b, w, s);
+    return build_sorted_cells(b, w, s, true);
 }
 
 // ------------------------------------------ reference-layout bucket tables --

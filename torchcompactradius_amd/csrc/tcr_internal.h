@@ -228,7 +228,7 @@ int build_structure(const Batch &b, Work &w, hipStream_t s);
 // Its stages: hMax + grid of every cloud (k_prep), and -- after the keys
 // (keys0/vals0/rwrap/pflags) are written -- sort, cells and lin table.
 int build_prep(const Batch &b, Work &w, hipStream_t s);
-int build_sorted_cells(const Batch &b, Work &w, hipStream_t s);
+int build_sorted_cells(const Batch &b, Work &w, hipStream_t s, bool hist_done = false);
 int cloud_cells_launch(const Batch &b, Work &w, hipStream_t s);
 
 // Multi-level search of one cloud (tcr_mlm.hip): level keys, group table,
@@ -250,7 +250,9 @@ struct CsrOut {
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted
 // positions gathered by its last pass.
-int radix_sort(const Batch &b, Work &w, hipStream_t s);
+// hist_done: the digit histograms (and the pass count) were built by the
+// compact path's keys kernel; else k_sort_hist builds them.
+int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done = false);
 
 // Decoupled look-back state: a 64-bit word per (tile[, digit]) holding a
 // 32-bit count and a flag -- 0 not published, aggregate of the tile alone, or
@@ -342,6 +344,62 @@ __device__ __forceinline__ CellIndex cell_index(const CloudDev &cd, const uint64
     ci.dense = cell_index_dense(cd);
     ci.empty = cd.h.nr == 0;
     return ci;
+}
+
+// ---------------------------------------------------------- sort tiles --
+// Sort tiles: kSortTile consecutive reference points of one cloud; a cloud's
+// tiles are numbered consecutively from t_start[c].
+
+struct TileInfo {
+    int c;           // cloud
+    int64_t start;   // first global key of the tile
+    int32_t len;     // keys in the tile
+    int64_t first;   // first tile of the cloud
+};
+
+__device__ __forceinline__ TileInfo tile_info(int64_t t, int32_t B, const int64_t *__restrict__ t_start,
+                                              const int64_t *__restrict__ p_start) {
+    // last cloud whose first tile <= t (clouds without tiles share their
+    // successor's first tile and are skipped by taking the last match)
+    const int c = find_segment(B, t, [&](int k) { return t_start[k]; });
+    TileInfo ti;
+    ti.c = c;
+    ti.first = t_start[c];
+    const int64_t lt = t - t_start[c];
+    const int64_t n = p_start[c + 1] - p_start[c];
+    ti.start = p_start[c] + lt * kSortTile;
+    const int64_t rem = n - lt * kSortTile;
+    ti.len = (int32_t)(rem < kSortTile ? rem : kSortTile);
+    return ti;
+}
+
+// Sort key: the int32 linear index in signed order.  A cloud whose keys all
+// lie in [0, prod numCells) < 2^31 sorts them unsigned (same order, fewer
+// passes); any other ("wide") cloud flips the sign bit and takes all passes.
+__device__ __forceinline__ uint32_t eff_key(uint32_t lin, bool wide) { return wide ? (lin ^ 0x80000000u) : lin; }
+
+// Points outside the grid (INT32_MIN coordinates) or more than 2^31 cells
+// (linear indices wrap to negative values).
+__device__ __forceinline__ bool cloud_wide(const CloudDev &cd) {
+    return cd.irregular || grid_cells(cd) > (1ull << 31);
+}
+
+// Radix passes a cloud needs.
+__device__ __forceinline__ uint32_t cloud_passes(const CloudDev &cd) {
+    if (cloud_wide(cd)) return kMaxPasses;
+    const uint64_t P = grid_cells(cd);
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < P) ++bits;
+    return (bits + kRadixBits - 1) / kRadixBits;
+}
+
+// Histograms count the digits of the raw linear key; a wide cloud sorts the
+// sign-flipped key, whose digit in the pass holding bit 31 is the raw digit
+// with that bit flipped.
+constexpr int kFlipPass = 31 / kRadixBits;
+constexpr uint32_t kFlipDigit = 1u << (31 % kRadixBits);
+__device__ __forceinline__ int hist_digit(int d, int pass, bool wide) {
+    return (wide && pass == kFlipPass) ? (int)((uint32_t)d ^ kFlipDigit) : d;
 }
 
 }  // namespace tcr

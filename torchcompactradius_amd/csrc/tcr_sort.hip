@@ -19,49 +19,6 @@
 
 namespace tcr {
 
-struct TileInfo {
-    int c;           // cloud
-    int64_t start;   // first global key of the tile
-    int32_t len;     // keys in the tile
-    int64_t first;   // first tile of the cloud
-};
-
-__device__ __forceinline__ TileInfo tile_info(int64_t t, int32_t B, const int64_t *__restrict__ t_start,
-                                              const int64_t *__restrict__ p_start) {
-    // last cloud whose first tile <= t (clouds without tiles share their
-    // successor's first tile and are skipped by taking the last match)
-    const int c = find_segment(B, t, [&](int k) { return t_start[k]; });
-    TileInfo ti;
-    ti.c = c;
-    ti.first = t_start[c];
-    const int64_t lt = t - t_start[c];
-    const int64_t n = p_start[c + 1] - p_start[c];
-    ti.start = p_start[c] + lt * kSortTile;
-    const int64_t rem = n - lt * kSortTile;
-    ti.len = (int32_t)(rem < kSortTile ? rem : kSortTile);
-    return ti;
-}
-
-// Sort key: the int32 linear index in signed order.  A cloud whose keys all
-// lie in [0, prod numCells) < 2^31 sorts them unsigned (same order, fewer
-// passes); any other ("wide") cloud flips the sign bit and takes all passes.
-__device__ __forceinline__ uint32_t eff_key(uint32_t lin, bool wide) { return wide ? (lin ^ 0x80000000u) : lin; }
-
-// Points outside the grid (INT32_MIN coordinates) or more than 2^31 cells
-// (linear indices wrap to negative values).
-__device__ __forceinline__ bool cloud_wide(const CloudDev &cd) {
-    return cd.irregular || grid_cells(cd) > (1ull << 31);
-}
-
-// Radix passes a cloud needs.
-__device__ __forceinline__ uint32_t cloud_passes(const CloudDev &cd) {
-    if (cloud_wide(cd)) return kMaxPasses;
-    const uint64_t P = grid_cells(cd);
-    uint32_t bits = 1;
-    while (bits < 32 && (1ull << bits) < P) ++bits;
-    return (bits + kRadixBits - 1) / kRadixBits;
-}
-
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restrict__ t_start,
             const int64_t *__restrict__ p_start, const uint32_t *__restrict__ keys, uint32_t *__restrict__ hist,
@@ -83,11 +40,10 @@ k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restr
             tab[g] = ~0ull;
     }
     __syncthreads();
-    const bool irr = cloud_wide(cd);
     for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
-        const uint32_t ek = eff_key(keys[ti.start + k], irr);
+        const uint32_t rk = keys[ti.start + k];  // raw key digits (hist_digit maps a wide cloud's)
 #pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p) atomicAdd(&h[p][(ek >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+        for (int p = 0; p < kMaxPasses; ++p) atomicAdd(&h[p][(rk >> (p * kRadixBits)) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;
@@ -109,6 +65,11 @@ struct SortPassArgs {
     uint64_t *lb;            // [ntiles * kRadix] of this pass
     uint32_t *misc;
     int pass;
+    // pass 0 empties the cell index when no k_sort_hist ran (the keys
+    // kernel built the histograms before the irregular flag was final)
+    int clear_index;
+    const int64_t *lt_start;
+    uint64_t *lintab;
     // last pass: gather
     const float4 *rwrap;
     const uint8_t *pflags;
@@ -139,6 +100,14 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     const int64_t tile = s_tile;
     const TileInfo ti = tile_info(tile, a.B, a.t_start, a.p_start);
     const bool irr = cloud_wide(a.desc[ti.c]);
+    if (a.clear_index && a.pass == 0) {
+        const CloudDev &cd = a.desc[ti.c];
+        const int64_t need = cell_index_dense(cd) ? (int64_t)((grid_cells(cd) + 1) / 2) : cd.lt_cap;
+        const int64_t ntiles = a.t_start[ti.c + 1] - ti.first;
+        uint64_t *tab = a.lintab + a.lt_start[ti.c];
+        for (int64_t g = (tile - ti.first) * kSortThreads + threadIdx.x; g < need; g += ntiles * kSortThreads)
+            tab[g] = ~0ull;
+    }
 
     uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -186,7 +155,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
             lb_store(lb, kLbInc | (excl + run));
             tile_excl[d] = (uint32_t)excl;
         }
-        dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + d];
+        dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + hist_digit(d, a.pass, irr)];
     }
     __syncthreads();
     // exclusive scan of the cloud's digit counts: wavefront scans of 64
@@ -232,13 +201,17 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     }
 }
 
-int radix_sort(const Batch &b, Work &w, hipStream_t s) {
+int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done) {
     if (b.rt_total == 0) return TCR_OK;
     // sort_hist and sort_lb were zeroed by build_prep's k_prep
-    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, (const CloudDev *)w.desc,
-                       b.B, (const int64_t *)w.rt_start, (const int64_t *)w.r_start, (const uint32_t *)w.keys0,
-                       w.sort_hist, w.misc, (const int64_t *)w.lt_start, w.lintab);
+    if (!hist_done)
+        hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s,
+                           (const CloudDev *)w.desc, b.B, (const int64_t *)w.rt_start, (const int64_t *)w.r_start,
+                           (const uint32_t *)w.keys0, w.sort_hist, w.misc, (const int64_t *)w.lt_start, w.lintab);
     SortPassArgs a;
+    a.clear_index = hist_done ? 1 : 0;
+    a.lt_start = w.lt_start;
+    a.lintab = w.lintab;
     a.desc = w.desc;
     a.B = b.B;
     a.ntiles = b.rt_total;
