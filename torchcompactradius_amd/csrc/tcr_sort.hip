@@ -3,8 +3,10 @@
 // (py/compactHashing/cellTable.py:33) on the reference's GPU path: inside a
 // cell, ascending original index.
 //
-// Onesweep layout: one histogram kernel counts the digits of every pass for
-// every cloud, then one kernel per pass ranks a tile of 4096 keys in LDS
+// Onesweep layout: the digits of every pass are counted per cloud first (by
+// the compact path's keys kernel, which writes the keys a tile at a time, or
+// by k_sort_hist here for the multi-level path's keys), then one kernel per
+// pass ranks a tile of 4096 keys in LDS
 // (16 wavefronts, 9-bit digits, wave64 multi-split with 9 ballots per
 // round), finds the tile's global
 // offset per digit by decoupled look-back over the earlier tiles of its
