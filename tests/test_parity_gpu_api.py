@@ -66,6 +66,45 @@ def test_batched_clouds_one_launch(seed):
     assert offs[-1] == sum(o.col.shape[0] for o in refs) == res.total
 
 
+def test_batched_empty_reference_and_non_finite():
+    """A batch mixing a query cloud whose reference set is empty, a
+    self-search cloud with NaN / inf coordinates, and a regular cloud: each
+    cloud's rows equal its own oracle search."""
+    rng = np.random.default_rng(77)
+    specs, refs = [], []
+    cases = []
+    q, hq, _ = _cloud(rng, 300, 2, [True, False], 0.08)
+    cases.append((q, hq, np.zeros((0, 2), np.float32), np.zeros(0, np.float32), [True, False], "gather"))
+    q, hq, _ = _cloud(rng, 2000, 3, [True, True, False], 0.2)
+    q[7, 1] = np.nan
+    q[100, 0] = np.inf
+    q[200, :] = np.nan
+    cases.append((q, hq, None, None, [True, True, False], "symmetric"))
+    q, hq, _ = _cloud(rng, 1500, 2, [False, True], 0.08)
+    cases.append((q, hq, None, None, [False, True], "scatter"))
+    for q, hq, r, hr, per, mode in cases:
+        dim = q.shape[1]
+        mn, mx = [-1.0] * dim, [1.0] * dim
+        pm = sum(1 << d for d in range(dim) if per[d])
+        specs.append(CloudSpec(qpos=_t(q), qsup=_t(hq), rpos=_t(r), rsup=_t(hr), domain_min=mn, domain_max=mx,
+                               periodic=pm, mode=mode, hash_map_length=4096))
+        refs.append(oracle.radius_search(q, hq, r, hr, domain_min=mn, domain_max=mx, periodic=per, mode=mode,
+                                         hash_map_length=4096))
+    print("case: batched empty reference + non-finite", flush=True)
+    res = run_search(specs, want_rows=True)
+    torch.cuda.synchronize()
+    offs = res.offsets.cpu().numpy()
+    row, col = res.row.cpu().numpy(), res.col.cpu().numpy()
+    q0 = 0
+    for s, o in zip(specs, refs):
+        nq = s.qpos.shape[0]
+        a, b = offs[q0], offs[q0 + nq]
+        np.testing.assert_array_equal(row[a:b], o.row)
+        np.testing.assert_array_equal(col[a:b], o.col)
+        q0 += nq
+    assert offs[-1] == sum(o.col.shape[0] for o in refs) == res.total
+
+
 def test_radius_with_batch_ids():
     rng = np.random.default_rng(77)
     n, m = 2000, 1500
