@@ -263,3 +263,90 @@ def test_deterministic_repeat():
     a = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, L=1 << 16, fmt="csr")
     b = gpu_search(w.pos, w.sup, mn=w.domain_min, mx=w.domain_max, periodic=w.periodic, L=1 << 16, fmt="csr")
     assert torch.equal(a.indices, b.indices) and torch.equal(a.indptr, b.indptr)
+
+
+def _strays_in_dense_cells(periodic):
+    """A dense ball (cells with far more than 448 stencil candidates) and 60
+    points just below domain_min in x that land among the ball's cell
+    members: in a non-periodic domain their cell -1 aliases the ball's cell
+    (nx-1, cy-1, cz) through the linear index; in a periodic one they wrap
+    onto max once (the query copy) and onto min twice (the sorted copy), so
+    they sort into the ball's cell (0, cy, cz).  The domain [0.1, 0.7]^3 is
+    one where 0.1 - 1 ulp once-wraps onto exactly 0.7."""
+    rng = np.random.default_rng(77 + periodic)
+    bg = rng.uniform(-1, 1, (1500, 3))
+    c = np.array([-0.95, 0.1, 0.1]) if periodic else np.array([0.93, 0.1, 0.1])
+    ball = np.clip(c + rng.normal(0, 0.03, (2500, 3)), -0.999, 0.999)
+    k = 60
+    st = np.empty((k, 3))
+    st[:, 1:] = c[1:] + rng.uniform(-0.03, 0.03, (k, 2))
+    if not periodic:
+        st[:, 1] += 0.125
+    near = st.copy()
+    near[:, 0] = -0.97  # in-domain neighbours, so that the strays' rows are not empty
+    pts = np.concatenate([bg, ball, near, st])
+    q = (np.float32(0.1) + (pts + 1) * 0.3).astype(np.float32)
+    below = np.nextafter(np.float32(0.1), np.float32(0))
+    q[-k:, 0] = below - (0 if periodic else np.arange(k, dtype=np.float32) * np.float32(3e-8))
+    h = (0.3 * rng.uniform(0.08, 0.12, q.shape[0])).astype(np.float32)
+    return q, h, k
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("mode", ["gather", "superSymmetric"])
+def test_strays_among_big_cell_members(periodic, mode):
+    """Stray members of cells too big for the small-cell kernels are searched
+    by the per-query kernel (their rows were once left unwritten)."""
+    q, h, k = _strays_in_dense_cells(periodic)
+    mn, mx, per = [0.1] * 3, [0.7] * 3, [periodic] * 3
+    print(f"case: strays among big-cell members periodic={periodic} mode={mode}", flush=True)
+    o = oracle.radius_search(q, h, domain_min=mn, domain_max=mx, periodic=per, mode=mode, hash_map_length=4096)
+    assert o.counts.max() > 448 and o.counts[-k:].min() > 0
+    for fmt in ("coo", "csr"):
+        g = gpu_search(q, h, mn=mn, mx=mx, periodic=per, mode=mode, L=4096, fmt=fmt)
+        assert_same(g, o if fmt == "coo" else oracle.radius_search(
+            q, h, domain_min=mn, domain_max=mx, periodic=per, mode=mode, hash_map_length=4096, want_rows=False),
+            fmt)
+
+
+@pytest.mark.parametrize("case", [(2, (False, False), (-1.0, -1.0), (1.0, 1.0)),
+                                  (2, (True, True), (0.1, 0.1), (0.7, 0.7)),
+                                  (3, (False, False, False), (-1.0,) * 3, (1.0,) * 3),
+                                  (3, (True, False, True), (0.1, -1.0, 0.1), (0.7, 1.0, 0.7))])
+@pytest.mark.parametrize("mode", ["gather", "scatter"])
+def test_supports_one_ulp_from_the_distance(case, mode):
+    """Self-search over islands that each hold one pair (xi, xj) at exact
+    distance d (the exact-rational model of modDistance) and three copies of
+    the point whose support decides it, with supports one fp32 ulp below d,
+    d and one ulp above: the compact path's filtered estimate must hand every
+    such pair to the reference formula and decide it as the model does, in
+    periodic and non-periodic dimensions."""
+    import exact_arith as ex
+    dim, per, mn, mx = case
+    mn32, mx32 = np.asarray(mn, np.float32), np.asarray(mx, np.float32)
+    rng = np.random.default_rng(7100 + dim + 10 * sum(per) + len(mode))
+    n_side = 20 if dim == 2 else 8
+    xi, xj, ds = ex.island_pairs(rng, dim, list(per), mn32, mx32, n_side,
+                                 lambda p: oracle.wrap(p, mn32, mx32, list(per)))
+    n = xi.shape[0]
+    sup3 = np.array([ex.supports_around(d) for d in ds], np.float32)          # [n, 3]
+    small = np.array([np.float32(float(d)) * np.float32(0.25) for d in ds], np.float32)
+    # points: 3 copies of the deciding point, then its partner
+    decider, partner = (xi, xj) if mode == "gather" else (xj, xi)
+    pos = np.concatenate([np.repeat(decider, 3, axis=0), partner]).astype(np.float32)
+    sup = np.concatenate([sup3.reshape(-1), small]).astype(np.float32)
+    print(f"case: one-ulp supports dim={dim} periodic={per} mode={mode} pairs={n}", flush=True)
+    g = gpu_search(pos, sup, mn=mn32, mx=mx32, periodic=list(per), mode=mode, L=8192)
+    o = oracle.radius_search(pos, sup, domain_min=mn32, domain_max=mx32, periodic=list(per), mode=mode,
+                             hash_map_length=8192)
+    assert_same(g, o)
+    row, col = g.row.cpu().numpy(), g.col.cpu().numpy()
+    pairs = set(zip(row.tolist(), col.tolist()))
+    got = np.zeros((n, 3), bool)
+    for t in range(n):
+        for k in range(3):
+            q, r = (3 * t + k, 3 * n + t)
+            got[t, k] = ((q, r) in pairs) if mode == "gather" else ((r, q) in pairs)
+    want = np.array([[ex.inside(ds[t], sup3[t, k]) for k in range(3)] for t in range(n)])
+    np.testing.assert_array_equal(got, want)
+    assert want[:, 2].all() and not want[:, :2].any()

@@ -359,6 +359,18 @@ __device__ __forceinline__ void append_strays(const GroupArgs &a, const CellView
     }
 }
 
+// The members of a fast cell whose stray bit is set (a query whose
+// once-wrapped position differs from its twice-wrapped sorted copy) go to
+// the per-query list; the cell kernels skip them.
+__device__ __forceinline__ void append_flagged(const GroupArgs &a, const CellView &v) {
+    for (uint32_t s = v.gb + __lane_id(); s < v.ge; s += kWave) {
+        if (a.sflags[s]) {
+            const uint32_t k = atomicAdd(a.stray_count, 1u);
+            a.stray_list[k] = (int32_t)(v.qs + a.sort_index[s]);
+        }
+    }
+}
+
 // Geometry of the filter for one cell.
 template <int DIM>
 struct Geo {
@@ -718,7 +730,7 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
             if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
         }
         for (uint32_t s = v.gb; s < v.ge; ++s) {
-            if (a.sflags[s]) continue;  // strays were listed by the small kernel
+            if (a.sflags[s]) continue;  // listed for the per-query kernel by k_count_small
             const int32_t qloc = uniform(a.sort_index[s]);
             const int64_t qg = v.qs + qloc;
             const float4 mq = a.sorted[s];
@@ -805,6 +817,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
                 append_strays(a, v);
             } else if (v.total > kSmallChunk) {
                 if (__lane_id() == 0) a.big_list[atomicAdd(a.big_count, 1u)] = cid;
+                append_flagged(a, v);  // k_group_big skips them
             } else {
                 count_small<DIM, MODE>(a, v, cid, s_slot[threadIdx.x / kWave]);
             }

@@ -124,8 +124,13 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const Tin *__restri
     }
 }
 
-// indptr of an empty query set: [0]
-__global__ void k_zero_indptr(int32_t *indptr) { indptr[0] = 0; }
+// Outputs of an empty query set: total 0 and indptr [0] -- a kernel, not a
+// memset, so that a captured graph holds no memset node.
+template <typename Tout>
+__global__ void k_zero_empty(Tout *total, int32_t *indptr) {
+    if (total) total[0] = 0;
+    if (indptr) indptr[0] = 0;
+}
 
 size_t scan_partials_needed(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 2; }
 
@@ -134,9 +139,9 @@ int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64
                    hipStream_t s, int32_t *indptr, int64_t *row_entries) {
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 0) {
-        if (write_total) TCR_HIP(hipMemsetAsync(out, 0, sizeof(Tout), s));
-        if (indptr) {
-            hipLaunchKernelGGL(k_zero_indptr, dim3(1), dim3(1), 0, s, indptr);
+        if (write_total || indptr) {
+            hipLaunchKernelGGL(k_zero_empty<Tout>, dim3(1), dim3(1), 0, s, write_total ? out : (Tout *)nullptr,
+                               indptr);
             TCR_LAUNCH_CHECK();
         }
         return TCR_OK;
