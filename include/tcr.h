@@ -290,6 +290,23 @@ int tcr_small_fill(const float *x, const float *hx, int64_t nx, const float *y, 
                    size_t workspace_bytes, const int64_t *offsets, void *ii, void *jj, int32_t index_bytes,
                    void *stream);
 
+/* The search of tcr_search_launch + tcr_search_finish without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a 64 MiB device array inside the library, one per GPU, held
+ * until the process ends (enough for about 140,000 points of a 3-D
+ * self-search); a search whose tcr_search_workspace_size exceeds it is
+ * rejected with TCR_ERR_SPACE.  Synchronises the stream once: writes the
+ * pair total to *total_pairs and, when it fits in `capacity`, the pairs as
+ * tcr_search_launch does (else TCR_ERR_SPACE, outputs not filled: call again
+ * with larger outputs).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other threads or streams) race. */
+int tcr_search_noalloc(const tcr_cloud *clouds, int32_t n_clouds,
+                       const float *qpos, const float *qsup,
+                       const float *rpos, const float *rsup,
+                       int64_t *offsets, void *row, void *col, int32_t index_bytes,
+                       int64_t capacity, int64_t *total_pairs, void *stream);
+
 /* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
  * (no counterpart in the reference; a benchmark hook).  While enabled, each
  * launch records HIP events on its stream that delimit the count phase

@@ -63,7 +63,7 @@ EXPORTED = (
     "tcr_mlm_workspace_size", "tcr_mlm_count", "tcr_mlm_fill", "tcr_csr_arrays",
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_naive_workspace_size", "tcr_naive_count", "tcr_naive_fill", "tcr_small_count", "tcr_small_fill",
-    "tcr_last_error", "tcr_version",
+    "tcr_last_error", "tcr_version", "tcr_search_noalloc",
 )
 
 
@@ -130,6 +130,9 @@ def load() -> ctypes.CDLL:
     L.tcr_build_neighbor_list.restype = ctypes.c_int
     L.tcr_build_neighbor_list.argtypes = [vp, vp, i64, vp, vp, i64, i32, vp, vp, i64, vp, i64, vp, vp, i64, vp,
                                           ctypes.c_double, vp, vp, u8p, i32, i32, vp, vp, vp]
+    L.tcr_search_noalloc.restype = ctypes.c_int
+    L.tcr_search_noalloc.argtypes = [cl, i32, vp, vp, vp, vp, vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_int64),
+                                     vp]
     dp = ctypes.POINTER(ctypes.c_double)
     L.tcr_naive_workspace_size.restype = ctypes.c_size_t
     L.tcr_naive_workspace_size.argtypes = [i64, i64, i32]

@@ -334,7 +334,44 @@ static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *tot
     return TCR_OK;
 }
 
-static bool pair_out(void *row, void *col, int32_t index_bytes, PairOut &out) {
+// tcr_search_launch and the completion of tcr_search_finish on a workspace
+// the caller owns, without the pinned readback slot: the pair total and the
+// error word come back by two small device-to-host copies into stack
+// variables, so nothing here allocates device or pinned memory.  Returns
+// TCR_ERR_SPACE (total written) when the pairs exceed the capacity; the
+// outputs are then not filled.
+int search_complete(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup, const float *rpos,
+                    const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets, PairOut out, int64_t capacity,
+                    int64_t *total_pairs, hipStream_t s) {
+    Prepared p;
+    int rc = launch_search(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, offsets, out, capacity, s, p, false);
+    if (rc) return rc;
+    int64_t total = 0;
+    uint32_t err = 0;
+    TCR_HIP(hipMemcpyAsync(&total, offsets + p.b.nq_total, sizeof(total), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipMemcpyAsync(&err, p.w.misc + kMiscErr, sizeof(err), hipMemcpyDeviceToHost, s));
+    TCR_HIP(hipStreamSynchronize(s));
+    if (err) {
+        for (int32_t c = 0; c < p.b.B; ++c) {
+            CloudDev cd;
+            TCR_HIP(hipMemcpy(&cd, p.w.desc + c, sizeof(CloudDev), hipMemcpyDeviceToHost));
+            if (cd.err) {
+                set_error("cloud " + std::to_string(c) +
+                          ": a periodic dimension has no cells (support exceeds the domain)");
+                return TCR_ERR_DOMAIN;
+            }
+        }
+    }
+    if (total_pairs) *total_pairs = total;
+    if (total > capacity) {
+        set_error("the search found " + std::to_string(total) + " pairs, more than the capacity of " +
+                  std::to_string(capacity) + "; the outputs were not filled");
+        return TCR_ERR_SPACE;
+    }
+    return TCR_OK;
+}
+
+bool pair_out(void *row, void *col, int32_t index_bytes, PairOut &out) {
     if (index_bytes != 4 && index_bytes != 8) {
         set_error("index_bytes must be 4 or 8");
         return false;

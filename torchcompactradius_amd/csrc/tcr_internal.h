@@ -246,6 +246,16 @@ struct CsrOut {
     int32_t *indptr = nullptr;
     int64_t *row_entries = nullptr;
 };
+// The whole search of tcr_search_launch on the workspace `ws`, completed
+// (one synchronisation) without allocating: pair total to *total_pairs,
+// TCR_ERR_DOMAIN for a degenerate grid, TCR_ERR_SPACE when the pairs exceed
+// the capacity (outputs not filled).
+int search_complete(const tcr_cloud *clouds, int32_t n, const float *qpos, const float *qsup, const float *rpos,
+                    const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets, PairOut out, int64_t capacity,
+                    int64_t *total_pairs, hipStream_t s);
+// Pair outputs from the C ABI's (row, col, index_bytes); false (error set)
+// for an index width other than 4 or 8.
+bool pair_out(void *row, void *col, int32_t index_bytes, PairOut &out);
 // Reference-layout bucket tables of every cloud (for export).
 int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // Segmented stable radix sort of the reference keys with the sorted
