@@ -7,12 +7,18 @@ inputs resident in HBM; the step runs radiusSearch(..., algorithm='compact',
 format='csr') end to end: structure build, count, scan and fill (into
 outputs sized from the previous step's total, so the chain runs without a
 host round trip in between), one host sync for the pair total, and the CSR
-tuple.  The roofline figure is the fill kernel (k_fill_small), timed with
-HIP events that the library records on the search stream around it during
-the timed steps.
+tuple.  The roofline figure is the dominant kernel of the search (the one
+with the larger average time of the two big passes, the wavefront-per-cell
+count kernel k_count_small and the fill kernel k_fill_small), each timed
+with HIP events that the library records on the search stream around it
+during the timed steps; both kernels' figures and the end-to-end fraction
+are reported beside it.
 
 Prints one JSON line (rank 0).  The CPU baseline is the C/OpenMP oracle
-timed on this host on the same workload.
+timed on this host on the same workload (BASELINE.md's protocol: one
+warm-up, median of five); the reference's own compiled count/fill ops
+(oracle/_ref) are timed beside it on the same structure where they were
+built.
 """
 from __future__ import annotations
 
@@ -40,24 +46,32 @@ def algorithmic_bytes(nq: int, dim: int, pairs: int, support_read: bool = True) 
     return nq * (4 * dim + (4 if support_read else 0)) + pairs * 8 + 4 * (nq + 1) + 8 * nq
 
 
-def fill_bytes(nq: int, dim: int, pairs: int) -> int:
-    """Algorithmic bytes of the fill kernel: positions + supports read, int64
-    row offsets read, int64 column indices written."""
-    return nq * (4 * dim + 4) + 8 * (nq + 1) + pairs * 8
+def fill_bytes(nq: int, pairs: int, index_bytes: int = 8) -> int:
+    """Algorithmic bytes of the fill kernel: int64 row offsets read (it reads
+    no positions or supports: it replays the count pass's hit masks) and the
+    column indices written."""
+    return 8 * (nq + 1) + pairs * index_bytes
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_fill_periodic3d_1M.md")
+def count_bytes(nq: int, dim: int) -> int:
+    """Algorithmic bytes of the count kernel: positions + supports of the
+    (self-search) cloud read once, int32 counts written."""
+    return nq * (4 * dim + 4) + 4 * nq
 
 
-def measured_traffic():
-    """HBM bytes per fill-kernel launch from the committed rocprofv3 FETCH_SIZE /
+TRAFFIC_FILES = {"k_fill_small": os.path.join(ROOT, "profiles", "pmc_fill_periodic3d_1M.md"),
+                 "k_count_small": os.path.join(ROOT, "profiles", "pmc_count_periodic3d_1M.md")}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (line 'traffic_bytes_per_launch: N'), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[kernel]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(float(line.split(":", 1)[1]))
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -72,46 +86,127 @@ def copy_probe(torch):
 
 
 def _cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it (model name, family/model/
+    stepping, logical CPUs)."""
+    info = {}
+    n = 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    n += 1
+                elif k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, model "
+            f"{info.get('model', '?')}, stepping {info.get('stepping', '?')}, {info.get('cpu MHz', '?')} MHz, "
+            f"{n} logical CPUs visible)")
 
 
-def cpu_baseline(w, reps: int = 3, single=None):
-    """The C/OpenMP oracle on this host: the full workload on all cores the
-    process may use (at most 16), median of `reps`; plus one single-thread
-    search of `single` (a smaller cloud of the same density, so that the
-    single-thread leg stays a few seconds)."""
+def host_threads() -> int:
+    """Threads of the CPU legs: the cores this process may run on, capped by
+    OMP_NUM_THREADS where it is set (the GPU box sets it to its CPU share
+    per GPU, 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _median_after_warmup(fn, reps: int):
+    fn()
+    ts = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), out
+
+
+def cpu_baseline(w, reps: int = 5, single=None):
+    """The C/OpenMP oracle on this host: the full workload on host_threads()
+    threads, one warm-up then the median of `reps` searches; plus one
+    single-thread search of `single` (a smaller cloud of the same density, so
+    that the single-thread leg stays a few seconds).  Where oracle/_ref is
+    built, the reference's own compiled countNeighbors + buildNeighborList
+    (with its int32 offsets and the sortIndex gather) are timed the same way
+    on the oracle's structure of the same cloud."""
     import oracle
     oracle.build()
-    threads = max(1, min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1),
-                         int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+    threads = host_threads()
 
     def one(cloud, nthreads):
-        t0 = time.perf_counter()
         r = oracle.radius_search(cloud.pos, cloud.sup, domain_min=cloud.domain_min, domain_max=cloud.domain_max,
                                  periodic=cloud.periodic, mode=cloud.mode, hash_map_length=cloud.hash_map_length,
                                  want_rows=False, threads=nthreads)
-        return time.perf_counter() - t0, int(r.col.shape[0])
+        return int(r.col.shape[0])
 
-    ts = []
-    P = 0
-    for _ in range(reps):
-        dt, P = one(w, threads)
-        ts.append(dt)
-    t = statistics.median(ts)
+    t, P = _median_after_warmup(lambda: one(w, threads), reps)
     s1 = single if single is not None else w
-    t1, P1 = one(s1, 1)
-    return {"value": P / t, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"full workload ({w.pos.shape[0]} points, {P} pairs), median of {reps} searches, "
-                      f"{t * 1e3:.0f} ms each; single thread: {s1.pos.shape[0]} points of the same density, "
-                      f"{P1} pairs, {t1 * 1e3:.0f} ms",
-            "single_thread_value": P1 / t1, "cpu_model": _cpu_model()}
+    t0 = time.perf_counter()
+    P1 = one(s1, 1)
+    t1 = time.perf_counter() - t0
+    out = {"value": P / t, "unit": "pairs/s", "cores": threads, "kind": "port",
+           "sample": f"full workload ({w.pos.shape[0]} points, {P} pairs): one warm-up, then the median of {reps} "
+                     f"searches, {t * 1e3:.0f} ms each; single thread: {s1.pos.shape[0]} points of the same "
+                     f"density, {P1} pairs, {t1 * 1e3:.0f} ms",
+           "single_thread_value": P1 / t1, "cpu_model": _cpu_model()}
+    try:
+        from oracle import ref
+        if ref.available():
+            ref.set_threads(threads)
+            args = ref.prepare(w.pos, w.sup, domain_min=w.domain_min, domain_max=w.domain_max, periodic=w.periodic,
+                               mode=w.mode, hash_map_length=w.hash_map_length)
+            tr, res = _median_after_warmup(lambda: ref.run_ops(args), reps)
+            Pr = int(res[2].shape[0])
+            out["reference_ops"] = {
+                "value": Pr / tr, "unit": "pairs/s", "cores": threads, "kind": "reference",
+                "sample": f"the reference's compiled countNeighbors, int32 offsets, buildNeighborList and "
+                          f"sortIndex gather on the full workload's structure ({Pr} pairs; the structure build, "
+                          f"Python torch ops in the reference, is not timed): one warm-up, median of {reps}, "
+                          f"{tr * 1e3:.0f} ms each",
+                "same_pairs_as_port": Pr == P}
+    except Exception as e:  # the reference leg is optional
+        out["reference_ops"] = {"error": str(e)[:200]}
+    return out
+
+
+def timed_steps(L, step, steps: int):
+    """`steps` calls of step() between two device synchronisations, with the
+    library's HIP-event phase timing on.  Returns (ms per step, mean count
+    phase ms, mean fill kernel ms, mean count kernel ms) -- NaN for a kernel
+    that did not run."""
+    import torch
+    L.tcr_timing_collect(None, None, None, 0, None)
+    L.tcr_timing_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.tcr_timing_enable(0)
+    cms, fms, kms = ((ctypes.c_float * steps)() for _ in range(3))
+    got = ctypes.c_int32(0)
+    L.tcr_timing_collect(cms, fms, kms, steps, ctypes.byref(got))
+
+    def mean(a):
+        v = [x for x in a[:got.value] if x >= 0]
+        return statistics.mean(v) if v else float("nan")
+    return (t1 - t0) * 1e3 / steps, mean(cms), mean(fms), mean(kms)
+
+
+def kernel_line(name: str, nbytes: int, avg_ms: float, launches: int) -> dict:
+    """One kernel's roofline figures: algorithmic bytes per launch over its
+    average HIP-event time, against HBM peak, and its measured traffic."""
+    ach = nbytes / (avg_ms / 1e3) / 1e9 if avg_ms == avg_ms else float("nan")
+    return {"algorithmic_bytes": nbytes, "avg_ms": avg_ms, "achieved": ach, "frac": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic(name), "launches": launches}
 
 
 def bench_batched(torch, reps: int = 5, scale: float = 1.0):
@@ -328,31 +423,16 @@ def main():
     torch.cuda.synchronize()
     P = int(out.indices.numel())
 
-    # the library records HIP events on the search stream around the count
-    # phase and around the fill kernel of every search of the timed region
     L = _lib.load()
-    L.tcr_timing_collect(None, None, 0, None)
-    L.tcr_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    L.tcr_timing_enable(0)
-    cms = (ctypes.c_float * args.steps)()
-    fms = (ctypes.c_float * args.steps)()
-    got = ctypes.c_int32(0)
-    L.tcr_timing_collect(cms, fms, args.steps, ctypes.byref(got))
-    ms = (t1 - t0) * 1e3 / args.steps
-    pairs_per_s = P / (ms / 1e3)
-    fill = [x for x in fms[:got.value] if x >= 0]
-    count = [x for x in cms[:got.value] if x >= 0]
-    fill_ms = statistics.mean(fill) if fill else float("nan")
-    count_ms = statistics.mean(count) if count else float("nan")
+    ms, count_ms, fill_ms, ck_ms = timed_steps(L, step, args.steps)
     nq = w.pos.shape[0]
-    fb = fill_bytes(nq, 3, P)
-    achieved = fb / (fill_ms / 1e3) / 1e9
+    pairs_per_s = P / (ms / 1e3)
     e2e_bytes = algorithmic_bytes(nq, 3, P)
+    kernels = {
+        "k_count_small": kernel_line("k_count_small", count_bytes(nq, 3), ck_ms, args.steps),
+        "k_fill_small": kernel_line("k_fill_small", fill_bytes(nq, P), fill_ms, args.steps),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] if kernels[k]["avg_ms"] == kernels[k]["avg_ms"] else -1)
     if args.copy_probe:
         copy_probe(torch)
 
@@ -365,22 +445,16 @@ def main():
     for _ in range(args.warmup):
         step32()
     torch.cuda.synchronize()
-    L.tcr_timing_collect(None, None, 0, None)
-    L.tcr_timing_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step32()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    L.tcr_timing_enable(0)
-    got32 = ctypes.c_int32(0)
-    L.tcr_timing_collect(cms, fms, args.steps, ctypes.byref(got32))
-    fill32 = [x for x in fms[:got32.value] if x >= 0]
-    ms32 = (t1 - t0) * 1e3 / args.steps
-    fill32_ms = statistics.mean(fill32) if fill32 else float("nan")
-    fb32 = nq * 16 + 8 * (nq + 1) + 4 * P
+    ms32, _, fill32_ms, _ = timed_steps(L, step32, args.steps)
+    fb32 = fill_bytes(nq, P, 4)
     int32_line = {"ms_per_step": ms32, "pairs_per_s": P / (ms32 / 1e3), "fill_kernel_ms": fill32_ms,
                   "fill_GB/s": fb32 / (fill32_ms / 1e3) / 1e9, "fill_algorithmic_bytes": fb32}
+
+    dk = kernels[dominant]
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": dk["achieved"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": dk["frac"], "traffic": dk["traffic"],
+                "algorithmic_bytes": dk["algorithmic_bytes"], "avg_ms": dk["avg_ms"], "launches": dk["launches"],
+                "end_to_end_frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "kernels": kernels}
 
     line = {
         "metric": METRIC,
@@ -397,9 +471,7 @@ def main():
         "data": "synthetic (seeded jittered grid)",
         "config": {"workload": w.name, "points": nq, "dim": 3, "periodic": "xyz", "mode": w.mode,
                    "format": "csr", "hash_map_length": w.hash_map_length, "pairs": P},
-        "roofline": {"bound": "hbm", "kernel": "k_fill_small", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "algorithmic_bytes": fb, "avg_ms": fill_ms, "launches": len(fill)},
+        "roofline": roofline,
         "end_to_end": {"algorithmic_bytes": e2e_bytes, "GB/s": e2e_bytes / (ms / 1e3) / 1e9,
                        "frac": e2e_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "count_phase_ms": count_ms,
                        "fill_kernel_ms": fill_ms, "points_per_s": nq / (ms / 1e3)},

@@ -310,13 +310,15 @@ int tcr_search_noalloc(const tcr_cloud *clouds, int32_t n_clouds,
 /* Phase timing of tcr_search_launch / tcr_search_count on the calling thread
  * (no counterpart in the reference; a benchmark hook).  While enabled, each
  * launch records HIP events on its stream that delimit the count phase
- * (build, count, scan, up to the fill kernel) and the fill pass's main kernel
- * (the wavefront-per-cell fill, else the per-query fill).  tcr_timing_collect
- * waits for the recorded
- * events, writes up to max_entries milliseconds per launch (-1 for a phase
- * that did not run), the number written to *n_entries, and forgets them. */
+ * (build, count, scan, up to the fill kernel), the first wavefront-per-cell
+ * count kernel, and the fill pass's main kernel (the wavefront-per-cell
+ * fill, else the per-query fill).  tcr_timing_collect waits for the recorded
+ * events, writes up to max_entries milliseconds per launch into each
+ * non-NULL array (-1 for a phase or kernel that did not run), the number
+ * written to *n_entries, and forgets them. */
 int tcr_timing_enable(int32_t on);
-int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int32_t *n_entries);
+int tcr_timing_collect(float *count_ms, float *fill_ms, float *count_kernel_ms, int32_t max_entries,
+                       int32_t *n_entries);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *tcr_last_error(void);

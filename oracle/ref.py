@@ -137,11 +137,12 @@ def set_threads(n: int) -> None:
     lib().ref_set_threads(int(n))
 
 
-def search(qpos, qsup, rpos=None, rsup=None, *, domain_min=None, domain_max=None, periodic=None, mode="gather",
-           hash_map_length=4096, search_range=1):
-    """The reference's count / offsets / fill / sortIndex sequence
-    (cppSearch.py:60-98) over the oracle's structure of the same inputs.
-    Returns (counts int32[nq], i int64[P], j int64[P])."""
+def prepare(qpos, qsup, rpos=None, rsup=None, *, domain_min=None, domain_max=None, periodic=None, mode="gather",
+            hash_map_length=4096, search_range=1):
+    """The op arguments of a search: the oracle's structure of the inputs
+    (sorted positions wrapped twice, cppSearch.py:221), the query positions
+    wrapped once (neighborhood.py:202-203) and the sort index.  Returns a
+    dict for ``run_ops``."""
     import oracle
 
     qpos = np.ascontiguousarray(qpos, np.float32)
@@ -164,11 +165,28 @@ def search(qpos, qsup, rpos=None, rsup=None, *, domain_min=None, domain_max=None
     si = o.sort_index.astype(np.int64)
     st = Structure(r2[si], None if rsup is None else np.asarray(rsup, np.float32)[si], o.hash_table,
                    o.cell_table, o.num_cells, o.hcell, mn, mx, per)
-    qs = None if qsup is None else np.asarray(qsup, np.float32)
-    counts = count_neighbors(st, q1, qs, mode, search_range)
+    qs = None if qsup is None else np.ascontiguousarray(qsup, np.float32)
+    return {"st": st, "q1": np.ascontiguousarray(q1), "qs": qs, "si": si, "mode": mode,
+            "search_range": search_range}
+
+
+def run_ops(a):
+    """countNeighbors, the int32 offsets of its cumsum, buildNeighborList and
+    j = sortIndex[j] (cppSearch.py:60-98) on prepared arguments.  Returns
+    (counts int32[nq], i int64[P], j int64[P])."""
+    st, q1, qs, si, mode, sr = a["st"], a["q1"], a["qs"], a["si"], a["mode"], a["search_range"]
+    nq = q1.shape[0]
+    counts = count_neighbors(st, q1, qs, mode, sr)
     offs = np.zeros(nq, np.int32)
     if nq > 1:
         offs[1:] = np.cumsum(counts, dtype=np.int64)[:-1].astype(np.int32)
     length = int(offs[-1]) + int(counts[-1]) if nq else 0
-    i, j = build_neighbor_list(st, counts, offs, length, q1, qs, mode, search_range)
+    i, j = build_neighbor_list(st, counts, offs, length, q1, qs, mode, sr)
     return counts, i, si[j] if length else j
+
+
+def search(qpos, qsup, rpos=None, rsup=None, **kw):
+    """The reference's count / offsets / fill / sortIndex sequence
+    (cppSearch.py:60-98) over the oracle's structure of the same inputs.
+    Returns (counts int32[nq], i int64[P], j int64[P])."""
+    return run_ops(prepare(qpos, qsup, rpos, rsup, **kw))

@@ -117,7 +117,7 @@ def load() -> ctypes.CDLL:
     L.tcr_timing_enable.restype = ctypes.c_int
     L.tcr_timing_enable.argtypes = [i32]
     L.tcr_timing_collect.restype = ctypes.c_int
-    L.tcr_timing_collect.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_int32)]
+    L.tcr_timing_collect.argtypes = [vp, vp, vp, i32, ctypes.POINTER(ctypes.c_int32)]
     L.tcr_search_finish.restype = ctypes.c_int
     L.tcr_search_finish.argtypes = [cl, i32, vp, ctypes.c_size_t, vp, ctypes.POINTER(ctypes.c_int64), vp]
     L.tcr_search_export.restype = ctypes.c_int

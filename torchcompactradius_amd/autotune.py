@@ -25,7 +25,7 @@ STENCIL_CANDIDATES = (0, 512, 1024, 4096)
 def _time_plan(plan, tuning: Tuple[int, int], reps: int) -> float:
     L = _lib.load()
     plan.tuning = tuning
-    L.tcr_timing_collect(None, None, 0, None)
+    L.tcr_timing_collect(None, None, None, 0, None)
     L.tcr_timing_enable(1)
     try:
         for _ in range(reps):
@@ -36,7 +36,7 @@ def _time_plan(plan, tuning: Tuple[int, int], reps: int) -> float:
     cms = (ctypes.c_float * n)()
     fms = (ctypes.c_float * n)()
     got = ctypes.c_int32(0)
-    L.tcr_timing_collect(cms, fms, n, ctypes.byref(got))
+    L.tcr_timing_collect(cms, fms, None, n, ctypes.byref(got))
     tot = [max(c, 0.0) + max(f, 0.0) for c, f in zip(cms[:got.value], fms[:got.value])]
     return statistics.median(tot) if tot else float("inf")
 

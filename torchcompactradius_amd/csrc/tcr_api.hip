@@ -225,7 +225,7 @@ __global__ void k_readback(const int64_t *__restrict__ total, const uint32_t *__
 // Optional per-thread phase timing: HIP events recorded on the search
 // stream around the count phase (build + count + scan) and the fill phase.
 struct PhaseEvents {
-    hipEvent_t e[4];
+    hipEvent_t e[6];  // 0 start, 1 end of the count phase / fill start, 3 fill end, 4-5 the count kernel
     bool has_fill;
 };
 struct Timing {
@@ -254,6 +254,8 @@ static hipEvent_t timing_event(PhaseEvents &pe, int k, hipStream_t s) {
 
 static thread_local KernelEvents g_fill_events;
 KernelEvents &fill_events() { return g_fill_events; }
+static thread_local KernelEvents g_count_events;
+KernelEvents &count_events() { return g_count_events; }
 
 // Everything of a search up to the scan, and the fill when capacity >= 0
 // (skipped on the device if the total exceeds it).  Asynchronous.
@@ -272,7 +274,22 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     if (timed) timing_event(pe, 0, s);
     rc = build_structure(p.b, p.w, s);
     if (rc) return rc;
+    KernelEvents &ce = count_events();
+    ce = KernelEvents{};
+    if (timed) {
+        pe.e[4] = pooled_event();
+        pe.e[5] = pooled_event();
+        ce.before = pe.e[4];
+        ce.after = pe.e[5];
+    }
     rc = query_count(p.b, p.w, s);
+    if (timed && !ce.recorded) {  // no wavefront-per-cell count kernel ran
+        if (pe.e[4]) hipEventRecord(pe.e[4], s);
+        if (pe.e[5]) hipEventRecord(pe.e[5], s);
+        pe.e[2] = pe.e[4];  // marks the count kernel as absent (e[2] is otherwise unused)
+        pe.e[4] = nullptr;
+    }
+    ce = KernelEvents{};
     if (rc) return rc;
     // the scan writes the CSR arrays too (before the fill, which the
     // device skips when the total exceeds the capacity)
@@ -553,16 +570,20 @@ int tcr_timing_enable(int32_t on) {
     return TCR_OK;
 }
 
-int tcr_timing_collect(float *count_ms, float *fill_ms, int32_t max_entries, int32_t *n_entries) {
+int tcr_timing_collect(float *count_ms, float *fill_ms, float *count_kernel_ms, int32_t max_entries,
+                       int32_t *n_entries) {
     int32_t k = 0;
     for (PhaseEvents &pe : g_timing.pending) {
         if (k < max_entries) {
-            float c = -1.0f, f = -1.0f;
+            float c = -1.0f, f = -1.0f, ck = -1.0f;
             if (pe.e[1] && hipEventSynchronize(pe.e[1]) == hipSuccess) hipEventElapsedTime(&c, pe.e[0], pe.e[1]);
             if (pe.has_fill && pe.e[1] && pe.e[3] && hipEventSynchronize(pe.e[3]) == hipSuccess)
                 hipEventElapsedTime(&f, pe.e[1], pe.e[3]);
+            if (pe.e[4] && pe.e[5] && hipEventSynchronize(pe.e[5]) == hipSuccess)
+                hipEventElapsedTime(&ck, pe.e[4], pe.e[5]);
             if (count_ms) count_ms[k] = c;
             if (fill_ms) fill_ms[k] = f;
+            if (count_kernel_ms) count_kernel_ms[k] = ck;
             ++k;
         }
         for (hipEvent_t e : pe.e)

@@ -921,9 +921,12 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
     const dim3 block(kGroupThreads), bgrid(256), lgrid(1024);
 #define TCR_L(D, M)                                                                                         \
     if (seen[D][M]) {                                                                                       \
-        if (!FILL)                                                                                          \
+        if (!FILL) {                                                                                        \
+            kernel_begin(count_events(), s);                                                                \
             hipLaunchKernelGGL((k_count_small<D, M>), dim3(group_grid(b, (const void *)k_count_small<D, M>)), \
                                block, 0, s, ga);                                                            \
+            kernel_end(count_events(), s);                                                                  \
+        }                                                                                                   \
         if (last == D * 4 + M)                                                                              \
             hipLaunchKernelGGL((k_group_big<D, M, FILL, true>), lgrid, block, 0, s, ga, gen);               \
         else                                                                                                \

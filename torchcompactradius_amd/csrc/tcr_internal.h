@@ -163,18 +163,19 @@ struct KernelEvents {
     hipEvent_t before = nullptr, after = nullptr;
     bool recorded = false;
 };
-KernelEvents &fill_events();  // this thread's
-inline void fill_kernel_begin(hipStream_t s) {
-    KernelEvents &k = fill_events();
+KernelEvents &fill_events();   // this thread's, around the fill pass's main kernel
+KernelEvents &count_events();  // this thread's, around the first wavefront-per-cell count kernel
+inline void kernel_begin(KernelEvents &k, hipStream_t s) {
     if (k.before && !k.recorded) (void)hipEventRecord(k.before, s);
 }
-inline void fill_kernel_end(hipStream_t s) {
-    KernelEvents &k = fill_events();
+inline void kernel_end(KernelEvents &k, hipStream_t s) {
     if (k.after && !k.recorded) {
         (void)hipEventRecord(k.after, s);
         k.recorded = true;
     }
 }
+inline void fill_kernel_begin(hipStream_t s) { kernel_begin(fill_events(), s); }
+inline void fill_kernel_end(hipStream_t s) { kernel_end(fill_events(), s); }
 
 // Word ranges a kernel zeroes on the side (k_prep: the sort and cell
 // look-back state of the search).
