@@ -194,6 +194,59 @@ int tcr_mlm_fill(const tcr_cloud *cloud, int32_t levels,
                  void *row, void *col, int32_t index_bytes, uint64_t *scratch,
                  const uint32_t *slots, int64_t slot_cap, void *stream);
 
+/* algorithm='mlm' with the reference's own semantics (a separate mode; the
+ * multi-level search above returns the compact result).  Replaces
+ * searchNeighbors_mlm with the hash-map cell table (py/multiLevelMemory/
+ * wrapper.py:145-308, cellOp.py:5-64, indexing.py:71-78, hashData.py:21-39,
+ * cpp/multiLevelMemory/mlmUtil.h:234-282, countNeighborsHashmap.h:99-124,
+ * buildNeighborhoodHashmap.h:103-128), quirks included: per-level codes
+ * shifted by 2 bits per level in every dimension, C-remainder neighbour
+ * cells with no periodic wrap, counts over the reference set at its
+ * unwrapped positions, superSymmetric support synchronisation, int32
+ * offsets of length nr.  Positions fp32 [n, dim] and supports fp32 [n] on
+ * the device; domain arrays on the HOST.
+ * 1. tcr_mlmref_plan reads the reference supports' range (synchronises),
+ *    fills *grid (computeResolutionLevels) and the workspace size; scratch:
+ *    >= 64 device bytes.  TCR_ERR_DOMAIN when no level exists (the
+ *    reference fails there).
+ * 2. tcr_mlmref_count builds the structure over the wrapped reference set,
+ *    counts over the unwrapped reference set and writes counts int32[nr]
+ *    (+ sync counts for superSymmetric), offsets int32[nr] (exclusive) and
+ *    the pair total (synchronises).  Not for scatter: the reference runs it
+ *    as the transposed gather search (tcr_mlmref_transpose).
+ * 3. tcr_mlmref_fill with the same arguments and workspace writes int64
+ *    (row, col) pairs, total entries (entries a row does not reach stay
+ *    (0, sortingIndices[0]), as the reference's zero-initialised list);
+ *    TCR_ERR_UNSUPPORTED where the reference's output is undefined (more
+ *    queries than reference points, or a query with more pairs than its
+ *    row). */
+typedef struct tcr_mlmref_grid {
+    int32_t dim, levels;
+    float hcell, hfine, hmin, hmax;
+    int32_t res[17][3];     /* cellResolutions per level */
+} tcr_mlmref_grid;
+int tcr_mlmref_plan(const float *rsup, int64_t nr, int64_t nq, int32_t dim, const float *domain_min,
+                    const float *domain_max, void *scratch, tcr_mlmref_grid *grid, size_t *workspace_bytes,
+                    void *stream);
+int tcr_mlmref_count(const tcr_mlmref_grid *grid, const float *qpos, const float *qsup, int64_t nq,
+                     const float *rpos, const float *rsup, int64_t nr, const float *domain_min,
+                     const float *domain_max, uint32_t periodic_mask, int32_t mode, int64_t hash_map_length,
+                     void *workspace, size_t workspace_bytes, int32_t *counts, int32_t *offsets, int64_t *total,
+                     void *stream);
+int tcr_mlmref_fill(const tcr_mlmref_grid *grid, const float *qpos, const float *qsup, int64_t nq,
+                    const float *rpos, const float *rsup, int64_t nr, const float *domain_min,
+                    const float *domain_max, uint32_t periodic_mask, int32_t mode, int64_t hash_map_length,
+                    void *workspace, size_t workspace_bytes, const int32_t *offsets, int64_t total, int64_t *row,
+                    int64_t *col, void *stream);
+/* transposeCOO(sorted=True) (py/multiLevelMemory/wrapper.py:119-142): the
+ * pairs (col, row) ordered stably by the new row; indptr int32[n_rows + 1]
+ * and row_entries int64[n_rows] (either may be NULL) are coo_to_csr's
+ * (py/util.py:335-358). */
+size_t tcr_mlmref_transpose_workspace_size(int64_t n, int64_t n_rows);
+int tcr_mlmref_transpose(const int64_t *row, const int64_t *col, int64_t n, int64_t n_rows, void *workspace,
+                         size_t workspace_bytes, int64_t *out_row, int64_t *out_col, int32_t *indptr,
+                         int64_t *row_entries, void *stream);
+
 /* Copy one cloud's structure out of the workspace in the reference layout:
  * sort_index int64[nr], sorted_pos fp32[nr, dim], sorted_sup fp32[nr] (NULL to
  * skip), cell_table int32[ncells, 3] (lin, begin, count) ordered by hash

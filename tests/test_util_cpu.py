@@ -37,3 +37,17 @@ def test_search_existing_variant_errors():
     for v in ("python", "pythonDynamic"):
         with pytest.raises(NotImplementedError):
             neighborSearchExisting(q, None, variant=v)
+
+
+def test_mlm_semantics_flag_validation():
+    """radiusSearch's mlmSemantics switch: unknown values and the dense
+    multi-level table (not built) raise before any device work."""
+    import pytest
+    import torch
+    from torchcompactradius_amd import PointCloud, radiusSearch
+    p = torch.zeros((4, 2))
+    h = torch.ones(4)
+    with pytest.raises(ValueError):
+        radiusSearch(PointCloud(p, h), algorithm="mlm", mlmSemantics="bogus")
+    with pytest.raises(ValueError):
+        radiusSearch(PointCloud(p, h), algorithm="mlm", mlmSemantics="reference", useDenseMLM=True)

@@ -41,6 +41,13 @@ class TcrTuning(ctypes.Structure):
     _fields_ = [("cell_workgroups", ctypes.c_int32), ("stencil_workgroups", ctypes.c_int32)]
 
 
+class TcrMlmrefGrid(ctypes.Structure):
+    """Mirror of ``tcr_mlmref_grid`` in include/tcr.h."""
+    _fields_ = [("dim", ctypes.c_int32), ("levels", ctypes.c_int32), ("hcell", ctypes.c_float),
+                ("hfine", ctypes.c_float), ("hmin", ctypes.c_float), ("hmax", ctypes.c_float),
+                ("res", (ctypes.c_int32 * 3) * 17)]
+
+
 class TcrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libtcr_hip error {code}: {msg}")
@@ -64,6 +71,8 @@ EXPORTED = (
     "tcr_compute_hash_indices", "tcr_count_neighbors", "tcr_build_neighbor_list",
     "tcr_naive_workspace_size", "tcr_naive_count", "tcr_naive_fill", "tcr_small_count", "tcr_small_fill",
     "tcr_last_error", "tcr_version", "tcr_search_noalloc",
+    "tcr_mlmref_plan", "tcr_mlmref_count", "tcr_mlmref_fill", "tcr_mlmref_transpose_workspace_size",
+    "tcr_mlmref_transpose",
 )
 
 
@@ -133,6 +142,19 @@ def load() -> ctypes.CDLL:
     L.tcr_search_noalloc.restype = ctypes.c_int
     L.tcr_search_noalloc.argtypes = [cl, i32, vp, vp, vp, vp, vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_int64),
                                      vp]
+    gp = ctypes.POINTER(TcrMlmrefGrid)
+    L.tcr_mlmref_plan.restype = ctypes.c_int
+    L.tcr_mlmref_plan.argtypes = [vp, i64, i64, i32, vp, vp, vp, gp, ctypes.POINTER(ctypes.c_size_t), vp]
+    L.tcr_mlmref_count.restype = ctypes.c_int
+    L.tcr_mlmref_count.argtypes = [gp, vp, vp, i64, vp, vp, i64, vp, vp, ctypes.c_uint32, i32, i64, vp,
+                                   ctypes.c_size_t, vp, vp, ctypes.POINTER(ctypes.c_int64), vp]
+    L.tcr_mlmref_fill.restype = ctypes.c_int
+    L.tcr_mlmref_fill.argtypes = [gp, vp, vp, i64, vp, vp, i64, vp, vp, ctypes.c_uint32, i32, i64, vp,
+                                  ctypes.c_size_t, vp, i64, vp, vp, vp]
+    L.tcr_mlmref_transpose_workspace_size.restype = ctypes.c_size_t
+    L.tcr_mlmref_transpose_workspace_size.argtypes = [i64, i64]
+    L.tcr_mlmref_transpose.restype = ctypes.c_int
+    L.tcr_mlmref_transpose.argtypes = [vp, vp, i64, i64, vp, ctypes.c_size_t, vp, vp, vp, vp, vp]
     dp = ctypes.POINTER(ctypes.c_double)
     L.tcr_naive_workspace_size.restype = ctypes.c_size_t
     L.tcr_naive_workspace_size.argtypes = [i64, i64, i32]

@@ -45,6 +45,7 @@ def radiusSearch(
         hashMapLengthAlgorithm: str = "primes",
         *,
         indexDtype: torch.dtype = torch.int64,
+        mlmSemantics: str = "compact",
 ) -> Union[SparseCOO, SparseCSR, Tuple[Union[SparseCOO, SparseCSR], CompactHashMap]]:
     """Fixed-radius neighbour search (neighborhood.py:136-309).
 
@@ -66,7 +67,15 @@ def radiusSearch(
     and 'mlm', ``supportOverride`` maps a scalar support onto every point (the
     reference's fixed-support compact variant is not callable).
     ``indexDtype`` (keyword only, an extension) selects int64 row/col
-    indices -- the reference's -- or int32 ones.
+    indices -- the reference's -- or int32 ones.  ``mlmSemantics``
+    (keyword only): ``'compact'`` (default) as above; ``'reference'`` runs
+    'mlm' with the reference's own multi-level semantics, quirks included
+    (searchNeighbors_mlm, multiLevelMemory/wrapper.py:145-308): counts over
+    the reference set, symmetric as gather, superSymmetric support
+    synchronisation, no periodic cell stepping, level codes shifted by two
+    bits per level (pairs go missing beyond level 0 in 1-D and 3-D), a CSR
+    whose indptr is the length-Nr offsets array; ``returnStructure`` gives
+    ``(result, None)`` and ``useDenseMLM`` is not built (ValueError).
     """
     with torch.no_grad():
         assert algorithm in ALGORITHMS, f"algorithm = {algorithm} not supported"
@@ -75,6 +84,11 @@ def radiusSearch(
         assert queryPointCloud is not None, f"referencePointCloud = {queryPointCloud} is None"
         if supportOverride is not None and not isinstance(supportOverride, float):
             raise ValueError(f"supportOverride = {supportOverride} must be a float")
+        if mlmSemantics not in ("compact", "reference"):
+            raise ValueError(f"mlmSemantics = {mlmSemantics} must be 'compact' or 'reference'")
+        if algorithm == "mlm" and mlmSemantics == "reference" and useDenseMLM:
+            raise ValueError("useDenseMLM: the dense multi-level cell table is not built; "
+                             "useDenseMLM=False runs the hash-map table")
         dim = queryPointCloud.positions.shape[1]
         if referencePointCloud is not None:
             assert referencePointCloud.positions.shape[1] == dim, \
@@ -108,6 +122,13 @@ def radiusSearch(
         if verbose:
             print(f"radiusSearch(compact): nq={qpos.shape[0]} nr={qpos.shape[0] if rpos is None else rpos.shape[0]} "
                   f"dim={dim} mode={mode} periodic_mask={pmask} hashMapLength={hashMapLength} format={format}")
+        if algorithm == "mlm" and mlmSemantics == "reference":
+            if qsup is None or (not self_search and rsup is None):
+                raise ValueError("the multi-level search reads the supports of both point clouds")
+            from .search import run_mlm_reference
+            res = run_mlm_reference(qpos, qsup, qpos if self_search else rpos, qsup if self_search else rsup,
+                                    mn, mx, pmask, mode, int(hashMapLength), format)
+            return (res, None) if returnStructure else res
         spec = CloudSpec(qpos=qpos, qsup=qsup, rpos=rpos, rsup=rsup, domain_min=mn, domain_max=mx,
                          periodic=pmask, mode=mode, hash_map_length=int(hashMapLength))
         res = None

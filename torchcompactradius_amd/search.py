@@ -412,6 +412,64 @@ def run_mlm(spec: CloudSpec, want_rows: bool, index_dtype=torch.int64, max_level
     return SearchResult(offsets, row, col, P, ws, clouds)
 
 
+def run_mlm_reference(qpos: torch.Tensor, qsup: torch.Tensor, rpos: torch.Tensor, rsup: torch.Tensor,
+                      domain_min: Sequence[float], domain_max: Sequence[float], pmask: int, mode: str,
+                      hash_map_length: int, fmt: str):
+    """algorithm='mlm' with the reference's own semantics: searchNeighbors_mlm
+    (py/multiLevelMemory/wrapper.py:145-308, hash-map cells) on
+    libtcr_hip.so's tcr_mlmref_* entry points.  Returns the reference's
+    result tuple: SparseCOO(row, col, nq, nr) or -- its CSR quirk --
+    SparseCSR(col, int32 offsets of length nr, int32 row lengths, nq, nr);
+    scatter is the transposed gather search (CSR through coo_to_csr)."""
+    from .util import SparseCOO, SparseCSR
+    L = _lib.load()
+    stream = _stream()
+    dev = qpos.device
+    nq, dim = qpos.shape
+    nr = rpos.shape[0]
+    if mode == "scatter":  # wrapper.py:191-192
+        inner = run_mlm_reference(rpos, rsup, qpos, qsup, domain_min, domain_max, pmask, "gather",
+                                  hash_map_length, "coo")
+        P = int(inner.row.numel())
+        wsb = L.tcr_mlmref_transpose_workspace_size(P, nq)
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+        out_row = torch.empty(P, dtype=torch.int64, device=dev)
+        out_col = torch.empty(P, dtype=torch.int64, device=dev)
+        ip = re = None
+        if fmt == "csr":
+            ip = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+            re = torch.empty(nq, dtype=torch.int64, device=dev)
+        check(L.tcr_mlmref_transpose(ptr(inner.row) if P else None, ptr(inner.col) if P else None, P, nq, ptr(ws),
+                                     wsb, ptr(out_row) if P else None, ptr(out_col) if P else None, ptr(ip), ptr(re),
+                                     stream))
+        if fmt == "coo":
+            return SparseCOO(out_row, out_col, nq, nr)
+        return SparseCSR(out_col, ip, re, nq, nr)
+    mn = (ctypes.c_float * dim)(*[float(x) for x in domain_min[:dim]])
+    mx = (ctypes.c_float * dim)(*[float(x) for x in domain_max[:dim]])
+    scratch = torch.empty(64, dtype=torch.uint8, device=dev)
+    grid = _lib.TcrMlmrefGrid()
+    wsb = ctypes.c_size_t(0)
+    check(L.tcr_mlmref_plan(ptr(rsup), nr, nq, dim, mn, mx, ptr(scratch), ctypes.byref(grid), ctypes.byref(wsb),
+                            stream))
+    ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
+    counts = torch.empty(nr, dtype=torch.int32, device=dev)
+    offsets = torch.empty(nr, dtype=torch.int32, device=dev)
+    total = ctypes.c_int64(0)
+    args = (ctypes.byref(grid), ptr(qpos), ptr(qsup), nq, ptr(rpos), ptr(rsup), nr, mn, mx, pmask, MODES[mode],
+            int(hash_map_length), ptr(ws), wsb.value)
+    check(L.tcr_mlmref_count(*args, ptr(counts), ptr(offsets), ctypes.byref(total), stream))
+    if mode == "superSymmetric" and nq != nr:  # wrapper.py:245-246
+        raise ValueError("SuperSymmetric support mode only works for equal number of query and reference particles")
+    P = int(total.value)
+    row = torch.empty(P, dtype=torch.int64, device=dev)
+    col = torch.empty(P, dtype=torch.int64, device=dev)
+    check(L.tcr_mlmref_fill(*args, ptr(offsets), P, ptr(row) if P else None, ptr(col) if P else None, stream))
+    if fmt == "coo":
+        return SparseCOO(row, col, nq, nr)
+    return SparseCSR(col, offsets, counts, nq, nr)
+
+
 def run_naive(qpos: torch.Tensor, qsup: Optional[torch.Tensor], rpos: torch.Tensor, rsup: Optional[torch.Tensor],
               domain_min: Sequence[float], domain_max: Sequence[float], periodic: int, wraps: int, mode: str,
               fixed_support: Optional[float] = None, index_dtype=torch.int64, small: bool = False):
