@@ -277,7 +277,7 @@ def bench_served(torch, n_requests: int = 512, rates=(100000.0, 1500.0), runs: i
     """Served stream: Poisson arrivals of 3-D search requests from `runs`
     simulated SPH runs (sizes log-uniform 1e3..1e6, every step displaced by
     at most 0.1 h), admitted under a 200 GB HBM budget.  Continuous batching
-    (batched launches on 2 streams, resident per-run plans) against the same
+    (batched launches on 2 launch slots run in order, resident per-run plans) against the same
     stream served by per-request plain launches, one at a time; every
     request's result must be identical in both (fingerprints compared)."""
     from torchcompactradius_amd.serve import (GpuBackend, Scheduler, latency_stats, make_simulations,

@@ -5,12 +5,14 @@ host loop
 
 * moves arrived requests into a FIFO queue,
 * admits queued requests into a new batch while the estimated HBM footprint
-  of everything in flight stays under the budget and a launch slot (a HIP
-  stream) is free -- footprint = workspace of the request's points + the
+  of everything in flight stays under the budget and a launch slot (a set
+  of resident batch buffers) is free -- footprint = workspace of the request's points + the
   predicted pairs (from that simulation's previous step) x index bytes,
 * launches each batch asynchronously as one batched search (all clouds in
   one launch chain, outputs sized from the predictions, no host round trip
-  between count and fill) on its own stream, so batches overlap on the device,
+  between count and fill) on its slot's stream, ordered after the previous
+  batch: each batch fills the device, and the host completes one while the
+  next runs,
 * completes batches as their events fire, frees their budget, and records
   each request's latency (completion - arrival).
 
@@ -21,7 +23,8 @@ buffer.  Each simulation's positions stay resident on the device and advance
 in place; its previous pair count sizes the next step's outputs.
 
 The scheduling core (``Scheduler``) is independent of the device; the GPU
-backend (``GpuBackend``) runs batches through ``SearchPlan``.
+backend (``GpuBackend``) runs batches through the library's batched search
+entry points.
 """
 from __future__ import annotations
 
@@ -161,6 +164,15 @@ def poisson_requests(n_requests: int, rate_hz: float, sizes: Sequence[int], seed
 
 # ------------------------------------------------------------------ GPU ----
 
+def _mix64(x: int) -> int:
+    """splitmix64 finaliser: a seeded integer hash (the per-step shift of a
+    simulation's displacement field)."""
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
 @dataclass
 class Simulation:
     """One simulated SPH run: resident positions advanced in place by seeded
@@ -182,17 +194,26 @@ class Simulation:
     plan_busy: bool = False       # the resident plan is in flight (one use at a time)
     plan_out: object = None       # resident output buffer (plan_cap entries)
     max_disp: float = 0.0
+    disp: object = None           # seeded displacement field (built on the first step)
 
     def advance(self, torch) -> None:
-        """Move every point by a seeded displacement of length <= max_disp
-        (uniform direction, uniform radius fraction)."""
+        """Move every point by a seeded displacement of length <= max_disp:
+        the simulation's displacement field (uniform directions, uniform
+        radius fractions, drawn once from its seed) rolled over the points by
+        a seeded per-step shift, so each step moves each point differently
+        at the cost of one add.  The sum is a new tensor: the previous
+        positions stay intact as the snapshot of the request that saw them."""
         self.step += 1
-        g = torch.Generator(device=self.pos.device)
-        g.manual_seed(self.seed * 1000003 + self.step)
-        d = torch.randn(self.pos.shape, generator=g, device=self.pos.device, dtype=torch.float32)
-        d = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
-        r = torch.rand((self.pos.shape[0], 1), generator=g, device=self.pos.device, dtype=torch.float32)
-        self.pos.add_(d * r * self.max_disp)
+        n = self.pos.shape[0]
+        if self.disp is None:
+            g = torch.Generator(device=self.pos.device)
+            g.manual_seed(self.seed * 1000003)
+            d = torch.randn((2 * n, 3), generator=g, device=self.pos.device, dtype=torch.float32)
+            d = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
+            r = torch.rand((2 * n, 1), generator=g, device=self.pos.device, dtype=torch.float32)
+            self.disp = d * (r * self.max_disp)
+        shift = _mix64(self.seed * 1000003 + self.step) % (n + 1)
+        self.pos = self.pos + self.disp[shift:shift + n]
 
 
 def make_simulations(torch, n_runs: int = 64, n_min: float = 1e3, n_max: float = 1e6, seed: int = 5,
@@ -217,8 +238,9 @@ def make_simulations(torch, n_runs: int = 64, n_min: float = 1e3, n_max: float =
 
 
 class GpuBackend:
-    """Runs batches of requests through batched search plans on ``slots``
-    HIP streams.  ``batched=False`` gives the per-request plain baseline:
+    """Runs batches of requests as batched searches on ``slots`` launch
+    slots -- a HIP stream and a set of resident batch buffers each, one batch
+    in flight per slot -- in launch order.  ``batched=False`` gives the per-request plain baseline:
     one radiusSearch per request, one at a time, on the current stream."""
 
     def __init__(self, torch, sims: Sequence[Simulation], slots: int = 2, batched: bool = True,
@@ -231,8 +253,17 @@ class GpuBackend:
         self.ib = 8 if self.index_dtype == torch.int64 else 4
         self.keep = keep_inputs
         self.resident_graph = resident_graph
+        # a slot's resident buffers (concatenated positions and supports,
+        # workspace, pinned first-pair readback) are reused by every batch it
+        # runs.  Each batch waits for the previous one's end event: a batch
+        # fills the device, so concurrent batches would only share it and
+        # finish together, idling it while the host completes them; in
+        # order, batch k+1 runs while the host completes batch k (whose
+        # stream synchronisation waits for batch k alone)
         self.streams = [torch.cuda.Stream() for _ in range(self.slots)] if batched else [None]
-        self._next = 0
+        self.free_slots = collections.deque(range(self.slots))
+        self.arena = [dict() for _ in range(self.slots)]
+        self.last_event = None
         self.pairs = 0
         self.alone = 0              # requests served by a resident graph replay
 
@@ -247,7 +278,7 @@ class GpuBackend:
         while s.step < r.step:
             s.advance(self.torch)
         r.predicted_pairs = max(r.predicted_pairs, s.last_pairs)
-        r.payload = s.pos.clone()
+        r.payload = s.pos      # advance() never writes it again: the snapshot
 
     def _spec(self, r: Request, pos):
         from .search import CloudSpec, periodic_mask
@@ -260,19 +291,31 @@ class GpuBackend:
         torch = self.torch
         if not self.batched:
             return None
-        stream = self.streams[self._next % self.slots]
-        self._next += 1
+        slot = self.free_slots.popleft()   # the scheduler keeps at most `slots` batches in flight
+        stream = self.streams[slot]
         stream.wait_stream(torch.cuda.current_stream())
+        if self.last_event is not None:
+            stream.wait_event(self.last_event)
+        # the requests hold their snapshots until complete() has synchronised
+        # this stream, so the snapshots need no record_stream
         with torch.cuda.stream(stream):
-            for r in b.requests:
-                r.payload.record_stream(stream)
             if len(b.requests) == 1 and not self.sims[b.requests[0].run].plan_busy:
                 handle = self._launch_alone(b.requests[0], stream)
             else:
-                handle = self._launch_batch(b, stream)
+                handle = self._launch_batch(b, stream, slot)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return (handle, ev, stream)
+        self.last_event = ev
+        return (handle, ev, stream, slot)
+
+    def _take(self, slot: int, name: str, n: int, dtype):
+        """A view of n elements of the slot's resident buffer `name`, grown
+        (by a quarter beyond the request) when too small."""
+        buf = self.arena[slot].get(name)
+        if buf is None or buf.numel() < n or buf.dtype != dtype:
+            buf = self.torch.empty(max(1, n + n // 4), dtype=dtype, device="cuda")
+            self.arena[slot][name] = buf
+        return buf[:n]
 
     def _launch_alone(self, r: Request, stream):
         """Resident plan of the simulation (workspace, offsets and output
@@ -305,16 +348,44 @@ class GpuBackend:
                                              s.plan.index_bytes, s.plan_cap, stream.cuda_stream))
         return ("alone", s.plan, s.plan_out, s.plan_cap)
 
-    def _launch_batch(self, b: Batch, stream):
-        from .search import SearchPlan
-        from ._lib import check, ptr
-        plan = SearchPlan([self._spec(r, r.payload) for r in b.requests], want_rows=False,
-                          index_dtype=self.index_dtype)
+    def _launch_batch(self, b: Batch, stream, slot: int):
+        """One batched search over the slot's resident buffers: the requests'
+        position snapshots and supports concatenated in place (one copy
+        kernel each), the cloud descriptors, the slot's workspace; fresh
+        outputs (the requests keep views of them)."""
+        from . import _lib
+        from ._lib import MODES, TcrCloud, check, ptr
+        from .search import periodic_mask
+        torch = self.torch
+        L = _lib.load()
+        n = len(b.requests)
+        clouds = (TcrCloud * n)()
+        q0 = 0
+        for k, r in enumerate(b.requests):
+            s = self.sims[r.run]
+            c = clouds[k]
+            c.dim, c.mode, c.periodic_mask, c.search_range = 3, MODES[s.mode], periodic_mask(s.periodic, 3), 1
+            for d in range(3):
+                c.domain_min[d], c.domain_max[d] = s.domain_min[d], s.domain_max[d]
+            c.hash_map_length, c.self_search = s.hash_map_length, 1
+            c.q_off, c.nq, c.q_foff, c.r_off, c.nr, c.r_foff = q0, r.n, 3 * q0, 0, r.n, 0
+            q0 += r.n
+        qpos = self._take(slot, "qpos", 3 * q0, torch.float32)
+        qsup = self._take(slot, "qsup", q0, torch.float32)
+        torch.cat([r.payload.reshape(-1) for r in b.requests], out=qpos)
+        torch.cat([self.sims[r.run].sup for r in b.requests], out=qsup)
+        wsb = L.tcr_search_workspace_size(clouds, n)
+        ws = self._take(slot, "ws", wsb, torch.uint8)
         cap = int(sum(r.predicted_pairs for r in b.requests) * 1.03) + 256
-        row, col = plan._alloc(cap)
-        check(plan.L.tcr_search_launch(*plan._args(), ptr(plan.offsets), None, ptr(col), plan.index_bytes, cap,
-                                       stream.cuda_stream))
-        return ("batch", plan, col, cap)
+        # offsets and pair indices in one allocation (the indices start on a
+        # 256-byte boundary): one allocator call per batch for its outputs
+        head = (q0 + 1 + 31) // 32 * 32
+        out = torch.empty(head + (cap * self.ib + 7) // 8, dtype=torch.int64, device="cuda")
+        offsets = out[:q0 + 1]
+        col = out[head:].view(self.index_dtype)[:cap]
+        args = (clouds, n, ptr(qpos), ptr(qsup), None, None, ptr(ws), wsb)
+        check(L.tcr_search_launch(*args, ptr(offsets), None, ptr(col), self.ib, cap, stream.cuda_stream))
+        return ("batch", args, offsets, col, cap)
 
     def ready(self, handle) -> bool:
         return handle is None or handle[1].query()
@@ -327,7 +398,7 @@ class GpuBackend:
         torch = self.torch
         if not self.batched:
             return self._serve_plain(b)
-        handle, ev, stream = b.handle
+        handle, ev, stream, slot = b.handle
         with torch.cuda.stream(stream):
             if handle[0] == "alone":
                 _, plan, col, cap = handle
@@ -339,20 +410,38 @@ class GpuBackend:
                 self._deliver(b.requests[0], plan.offsets.clone(), 0, col, P)
                 self.sims[b.requests[0].run].plan_busy = False
             else:
-                _, plan, col, cap = handle
-                P = plan._finish(stream.cuda_stream)
-                if P > cap:
-                    _, col = plan._fill_exact(P, stream.cuda_stream)
+                import ctypes
+                from . import _lib
+                from ._lib import check, ptr
+                L = _lib.load()
+                _, args, offsets, col, cap = handle
                 starts, q0 = [], 0
                 for r in b.requests:
                     starts.append(q0)
                     q0 += r.n
                 starts.append(q0)
-                idx = torch.tensor(starts, dtype=torch.int64).to(plan.offsets.device, non_blocking=True)
-                bounds = plan.offsets[idx].cpu().tolist()
+                # each request's first pair: gathered on the stream before the
+                # host waits, read after the one synchronisation
+                idx = torch.tensor(starts, dtype=torch.int64).to(offsets.device, non_blocking=True)
+                bounds_dev = offsets[idx]
+                bounds_host = self.arena[slot].get("bounds")
+                if bounds_host is None or bounds_host.numel() < len(starts):
+                    bounds_host = torch.empty(max(len(starts), 65), dtype=torch.int64, pin_memory=True)
+                    self.arena[slot]["bounds"] = bounds_host
+                bounds_host = bounds_host[:len(starts)]
+                bounds_host.copy_(bounds_dev, non_blocking=True)
+                total = ctypes.c_int64(0)
+                check(L.tcr_search_finish(args[0], args[1], args[6], args[7], ptr(offsets), ctypes.byref(total),
+                                          stream.cuda_stream))
+                P = int(total.value)
+                if P > cap:
+                    col = torch.empty(P, dtype=self.index_dtype, device="cuda")
+                    check(L.tcr_search_fill(*args, ptr(offsets), None, ptr(col), self.ib, stream.cuda_stream))
+                bounds = bounds_host.tolist()
                 for k, r in enumerate(b.requests):
                     a, e = bounds[k], bounds[k + 1]
-                    self._deliver(r, plan.offsets[starts[k]:starts[k + 1] + 1], a, col[a:e], e - a)
+                    self._deliver(r, offsets[starts[k]:starts[k + 1] + 1], a, col[a:e], e - a)
+        self.free_slots.append(slot)
         for r in b.requests:
             if self.keep:
                 r.inputs = r.payload
