@@ -379,6 +379,62 @@ def bench_adaptive(torch, n: int = 2_000_000, reps: int = 3):
                        "sampled_rows": int(rows.numel())}}
 
 
+def bench_nonself(torch, n: int = 100, reps: int = 5):
+    """Query set != reference set: 1M queries (a jittered grid of another
+    seed, shifted by half a grid step) against the 1M-point jittered grid,
+    uniform support for ~50 neighbours, gather mode, non-periodic, CSR --
+    the search radius() and neighborSearchExisting run.  Check: sampled rows
+    equal a brute-force evaluation of the reference predicate in the compact
+    order (stencil offset of the reference point's cell seen from the
+    query's cell, then index)."""
+    from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch, synthetic
+    from torchcompactradius_amd.util import compute_h
+    wr = synthetic.periodic_3d(n=n, seed=2)
+    wq = synthetic.periodic_3d(n=n, seed=7)
+    dx = 2.0 / n
+    qpos_np = ((wq.pos + np.float32(dx / 2) + np.float32(1.0)) % np.float32(2.0) - np.float32(1.0)).astype(np.float32)
+    rpos = torch.from_numpy(wr.pos).cuda()
+    rsup = torch.from_numpy(wr.sup).cuda()
+    qpos = torch.from_numpy(qpos_np).cuda()
+    qsup = torch.from_numpy(wq.sup).cuda()
+    dom = DomainDescription(torch.tensor([-1.0] * 3), torch.tensor([1.0] * 3), torch.tensor([False] * 3), 3)
+
+    def step():
+        return radiusSearch(PointCloud(qpos, qsup), PointCloud(rpos, rsup), mode="gather", domain=dom,
+                            hashMapLength=1 << 20, algorithm="compact", format="csr")
+    out = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    P = int(out.indices.shape[0])
+    # sampled rows against brute force (gather: dist < h of the query)
+    hcell = compute_h([-1.0] * 3, [1.0] * 3, float(wq.sup.max()))
+    rcells = torch.floor((rpos + 1.0) / hcell).to(torch.int64)
+    g = torch.Generator().manual_seed(5)
+    rows = torch.randint(0, qpos.shape[0], (64,), generator=g)
+    ok = True
+    N = rpos.shape[0]
+    indptr = out.indptr.to(torch.int64)
+    for i in rows.tolist():
+        d = qpos[i:i + 1] - rpos
+        acc = torch.zeros(N, dtype=torch.float32, device=rpos.device)
+        for k in range(3):
+            acc = (acc.double() + d[:, k].double() ** 2).float()
+        j = torch.nonzero(torch.sqrt(acc) < qsup[i]).reshape(-1)
+        qc = torch.floor((qpos[i] + 1.0) / hcell).to(torch.int64)
+        dc = rcells[j] - qc
+        o = (dc[:, 0] + 1) * 9 + (dc[:, 1] + 1) * 3 + (dc[:, 2] + 1)
+        want = j[torch.argsort(o * N + j)]
+        got = out.indices[int(indptr[i]):int(indptr[i + 1])].to(torch.int64)
+        ok = ok and bool(torch.equal(got, want))
+    return {"workload": f"nonself-3d-{qpos.shape[0]}x{N}", "queries": int(qpos.shape[0]), "references": N,
+            "pairs": P, "ms": t * 1e3, "pairs_per_s": P / t, "mode": "gather", "format": "csr",
+            "periodic": "none", "checks": {"sampled_rows_vs_brute_force": bool(ok), "sampled_rows": 64}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,9 +445,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (256-cloud) comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream comparison")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-support multi-level search")
+    ap.add_argument("--no-nonself", action="store_true", help="skip the query != reference search")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--only", choices=("batched", "served", "adaptive"),
+    ap.add_argument("--only", choices=("batched", "served", "adaptive", "nonself"),
                     help="run only this comparison leg and print its JSON object (to profile it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -401,7 +458,8 @@ def main():
 
     import torch
     if args.only:
-        leg = {"batched": bench_batched, "served": bench_served, "adaptive": bench_adaptive}[args.only]
+        leg = {"batched": bench_batched, "served": bench_served, "adaptive": bench_adaptive,
+               "nonself": bench_nonself}[args.only]
         print(json.dumps({"leg": args.only, **leg(torch)}), flush=True)
         return
     from torchcompactradius_amd import DomainDescription, PointCloud, _lib, radiusSearch, synthetic
@@ -483,6 +541,8 @@ def main():
         line["served"] = bench_served(torch)
     if not args.no_adaptive:
         line["adaptive"] = bench_adaptive(torch)
+    if not args.no_nonself:
+        line["nonself"] = bench_nonself(torch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, single=synthetic.periodic_3d(n=max(10, args.n // 2)))
     print(json.dumps(line), flush=True)

@@ -147,12 +147,16 @@ int tcr_graph_capture(const tcr_cloud *clouds, int32_t n_clouds,
 int tcr_graph_launch(void *graph, void *stream);
 int tcr_graph_destroy(void *graph);
 
-/* Launch shape of the wavefront-per-cell kernels, process-wide (set it
- * between searches, not while another thread launches one); 0 = automatic.
- * Picked per batch shape by the package's autotuner. */
+/* Launch shape of the searches the calling thread launches (each host
+ * thread has its own: threads that run differently tuned plans do not race);
+ * 0 = automatic.  A captured graph keeps the shape it was captured with.
+ * Picked per batch shape by the package's autotuner.  No counterpart in the
+ * reference (its launch shapes are fixed). */
 typedef struct tcr_tuning {
     int32_t cell_workgroups;     /* workgroups of the count/fill cell kernels (rounded up to a multiple of 8) */
     int32_t stencil_workgroups;  /* workgroups of the stencil resolution kernel */
+    int32_t cell_threads;        /* threads per workgroup of the cell kernels: 64, 128 or 256 (0: 256) */
+    int32_t cells_per_wave;      /* consecutive cells a wavefront takes in turn, 1..64 (0: 1) */
 } tcr_tuning;
 int tcr_set_tuning(const tcr_tuning *tuning);  /* NULL restores the defaults */
 int tcr_get_tuning(tcr_tuning *tuning);

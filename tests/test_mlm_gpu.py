@@ -169,3 +169,40 @@ def test_slot_capacity_and_overflow(cap, monkeypatch):
         if fmt_rows:
             np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
         np.testing.assert_array_equal(r.col.cpu().numpy().astype(np.int64), o.col)
+
+
+@pytest.mark.parametrize("case", ["four_periodic_cells", "wrap_strays", "dense_runs"])
+def test_multilevel_edge_inputs_match_oracle(case):
+    """Inputs at the edges of the multi-level traversal: a periodic dimension
+    of 4 hCell cells (neighbour columns wrap onto each other's images),
+    points just below the domain minimum whose once-wrapped copy (the query)
+    differs from the twice-wrapped one (the reference), and a dense core of
+    small supports inside a sparse cloud of large ones (groups searched by
+    sub-cell)."""
+    rng = np.random.default_rng({"four_periodic_cells": 901, "wrap_strays": 902, "dense_runs": 903}[case])
+    mn, mx = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+    if case == "four_periodic_cells":
+        n, per = 3000, [True, False, True]
+        pos = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+        sup = (0.12 * np.exp2(rng.uniform(0, 2, n))).astype(np.float32)
+        sup[0] = 0.5                                     # hMax 0.5: 4 cells per dimension
+    elif case == "wrap_strays":
+        n, per = 4000, [True, True, True]
+        pos = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+        pos[:200, 0] = np.float32(-1.0) - np.float32(1e-8) * rng.integers(1, 4, 200).astype(np.float32)
+        pos[200:400, 2] = np.nextafter(np.float32(-1.0), np.float32(-2.0))
+        sup = (0.05 * np.exp2(rng.uniform(0, 3, n))).astype(np.float32)
+    else:
+        n, per = 20000, [False, False, False]
+        c = rng.normal(0.0, 0.05, (n // 2, 3))
+        u = rng.uniform(-1, 1, (n - n // 2, 3))
+        pos = np.clip(np.concatenate([c, u]), -0.999, 0.999).astype(np.float32)
+        sup = np.where(np.abs(pos).max(axis=1) < 0.15, 0.02, 0.16).astype(np.float32)
+    print(f"case: mlm edge inputs {case}", flush=True)
+    for mode in ("symmetric", "superSymmetric", "scatter"):
+        r = run_mlm(_spec(pos, sup, mn, mx, per, mode), want_rows=True)
+        torch.cuda.synchronize()
+        o = oracle.radius_search(pos, sup, domain_min=mn, domain_max=mx, periodic=per, mode=mode,
+                                 hash_map_length=4096)
+        np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)

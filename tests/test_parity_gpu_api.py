@@ -370,7 +370,7 @@ def test_graph_replay_int32_and_tuning():
         h = np.full(n, 0.12 if dim == 2 else 0.25, np.float32)
         base.append([q, h, [bool(rng.integers(0, 2)) for _ in range(dim)], ["gather", "symmetric"][k % 2]])
     specs, refs = _specs_and_refs(rng, base)
-    for tuning in (None, (8, 64), (1024, 4096)):
+    for tuning in (None, (8, 64), (1024, 4096), (0, 0, 64, 4), (2048, 512, 128, 2)):
         for dt in (torch.int64, torch.int32):
             print(f"case: graph tuning={tuning} dtype={dt}", flush=True)
             plan = SearchPlan(specs, want_rows=False, index_dtype=dt)
@@ -389,3 +389,45 @@ def test_graph_replay_int32_and_tuning():
             plan.qpos.copy_(torch.cat([s.qpos.reshape(-1) for s in specs2]))
             _check_batch(plan.replay(), specs2, refs2)
             plan.release_graph()
+
+
+def test_threads_with_different_launch_shapes():
+    """Two host threads run differently tuned plans of the same batch at
+    the same time (the library keeps one launch shape per thread): both give
+    the oracle's pairs."""
+    import threading
+    from torchcompactradius_amd.search import SearchPlan
+    rng = np.random.default_rng(56)
+    base = []
+    for k in range(4):
+        dim = 2 + k % 2
+        n = int(rng.integers(2000, 6000))
+        q = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+        h = np.full(n, 0.1 if dim == 2 else 0.2, np.float32)
+        base.append([q, h, [bool(rng.integers(0, 2)) for _ in range(dim)], "gather"])
+    specs, refs = _specs_and_refs(rng, base)
+    print("case: two threads, two launch shapes", flush=True)
+    results, errors = {}, []
+
+    def work(name, tuning):
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                plan = SearchPlan(specs, want_rows=False)
+                plan.tuning = tuning
+                for _ in range(3):
+                    r = plan.run()
+                stream.synchronize()
+                results[name] = r
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    ts = [threading.Thread(target=work, args=("a", (8, 64, 64, 4))),
+          threading.Thread(target=work, args=("b", (2048, 0, 256, 1)))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for name in ("a", "b"):
+        _check_batch(results[name], specs, refs)

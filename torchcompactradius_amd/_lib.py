@@ -38,7 +38,8 @@ class TcrCloud(ctypes.Structure):
 
 class TcrTuning(ctypes.Structure):
     """Mirror of ``tcr_tuning`` in include/tcr.h."""
-    _fields_ = [("cell_workgroups", ctypes.c_int32), ("stencil_workgroups", ctypes.c_int32)]
+    _fields_ = [("cell_workgroups", ctypes.c_int32), ("stencil_workgroups", ctypes.c_int32),
+                ("cell_threads", ctypes.c_int32), ("cells_per_wave", ctypes.c_int32)]
 
 
 class TcrMlmrefGrid(ctypes.Structure):

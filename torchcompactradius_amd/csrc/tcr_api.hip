@@ -397,7 +397,9 @@ bool pair_out(void *row, void *col, int32_t index_bytes, PairOut &out) {
     return true;
 }
 
-static Tuning g_tuning;
+// Launch shape of the calling thread's searches (each host thread has its
+// own, so threads running differently tuned plans do not race).
+static thread_local Tuning g_tuning;
 Tuning &tuning() { return g_tuning; }
 
 }  // namespace tcr
@@ -542,8 +544,18 @@ int tcr_set_tuning(const tcr_tuning *t) {
         set_error("tuning values must be in [0, 2^20]");
         return TCR_ERR_ARG;
     }
+    if (t->cell_threads != 0 && t->cell_threads != 64 && t->cell_threads != 128 && t->cell_threads != 256) {
+        set_error("cell_threads must be 0, 64, 128 or 256");
+        return TCR_ERR_ARG;
+    }
+    if (t->cells_per_wave < 0 || t->cells_per_wave > 64) {
+        set_error("cells_per_wave must be in [0, 64]");
+        return TCR_ERR_ARG;
+    }
     g_tuning.cell_workgroups = t->cell_workgroups;
     g_tuning.stencil_workgroups = t->stencil_workgroups;
+    g_tuning.cell_threads = t->cell_threads;
+    g_tuning.cells_per_wave = t->cells_per_wave;
     return TCR_OK;
 }
 
@@ -554,6 +566,8 @@ int tcr_get_tuning(tcr_tuning *t) {
     }
     t->cell_workgroups = g_tuning.cell_workgroups;
     t->stencil_workgroups = g_tuning.stencil_workgroups;
+    t->cell_threads = g_tuning.cell_threads;
+    t->cells_per_wave = g_tuning.cells_per_wave;
     return TCR_OK;
 }
 

@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "tcr_generic.h"
 
@@ -157,6 +158,7 @@ struct GroupArgs {
     uint32_t *stray_count;
     uint32_t *big_list;          // [cells]
     uint32_t *big_count;
+    uint32_t cell_run;           // consecutive cells a wavefront takes in turn (tuned)
 };
 
 // ------------------------------------------------------------ stencil ----
@@ -778,21 +780,33 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
 // ------------------------------------------------------------- kernels ----
 
 // XCD-aware static schedule: workgroup b runs on XCD b % 8; each XCD takes a
-// contiguous range of cells so that neighbouring cells share its L2.
+// contiguous range of cells so that neighbouring cells share its L2.  Inside
+// an XCD's range the wavefronts take runs of `run` consecutive cells in
+// turn (run 1: wavefront w takes cells w, w + W, w + 2W, ...).
 struct CellRange {
-    uint32_t first, end, step;
+    uint32_t first, end, step, run;
+    // the cell after cid; k counts the cells taken of the current run
+    __device__ __forceinline__ uint32_t next(uint32_t cid, uint32_t &k) const {
+        if (k < run) {
+            ++k;
+            return cid + 1;
+        }
+        k = 1;
+        return cid + 1 + step - run;
+    }
 };
-__device__ __forceinline__ CellRange my_cells(uint32_t ncells) {
+__device__ __forceinline__ CellRange my_cells(uint32_t ncells, uint32_t run) {
     const uint32_t nxcd = 8;
-    const uint32_t wpb = kGroupThreads / kWave;
+    const uint32_t wpb = blockDim.x / kWave;
     const uint32_t xcd = blockIdx.x % nxcd;
     const uint32_t per_xcd_waves = (gridDim.x / nxcd) * wpb;
     const uint32_t wix = (blockIdx.x / nxcd) * wpb + threadIdx.x / kWave;
     CellRange r;
+    r.run = run > 0 ? run : 1u;
     const uint32_t c0 = (uint32_t)(((uint64_t)ncells * xcd) / nxcd);
     r.end = (uint32_t)(((uint64_t)ncells * (xcd + 1)) / nxcd);
-    r.first = c0 + wix;
-    r.step = per_xcd_waves;
+    r.first = c0 + wix * r.run;
+    r.step = per_xcd_waves * r.run;
     return r;
 }
 
@@ -805,12 +819,15 @@ __device__ __forceinline__ bool cloud_is(const CloudDev &cd, int dim, int mode) 
 template <int DIM, int MODE>
 __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) k_count_small(GroupArgs a) {
     constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
-    __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
-    const CellRange cr = my_cells(a.misc[kMiscCells]);
+    // per wavefront: the slot table (dynamic LDS: kSmallChunk words per
+    // wavefront of the block)
+    extern __shared__ int32_t s_dyn[];
+    int32_t *const s_slot = s_dyn + uniform(threadIdx.x / kWave) * kSmallChunk;
+    const CellRange cr = my_cells(a.misc[kMiscCells], a.cell_run);
     if (cr.first >= cr.end) return;
     // no next-cell prefetch here: its registers would cost the occupancy
     // that hides the same latency
-    for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
+    for (uint32_t cid = cr.first, k = 1; cid < cr.end; cid = cr.next(cid, k)) {
         const CellView v = make_view(a, load_cell(a, cid), kOff);
         if (v.state != kCellSkip && cloud_is(a.desc[v.c], DIM, MODE)) {
             if (v.state == kCellStray) {
@@ -819,7 +836,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
                 if (__lane_id() == 0) a.big_list[atomicAdd(a.big_count, 1u)] = cid;
                 append_flagged(a, v);  // k_group_big skips them
             } else {
-                count_small<DIM, MODE>(a, v, cid, s_slot[threadIdx.x / kWave]);
+                count_small<DIM, MODE>(a, v, cid, s_slot);
             }
         }
     }
@@ -828,12 +845,15 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
 // 8 waves per SIMD (<= 64 VGPRs, no spills)
 template <int KIND>
 __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) k_fill_small(GroupArgs a) {
-    __shared__ int32_t s_slot[kGroupThreads / kWave][kSmallChunk];
-    __shared__ int32_t s_row[kGroupThreads / kWave][kSmallChunk];
+    // per wavefront: the slot table and the row buffer (dynamic LDS:
+    // 2 * kSmallChunk words per wavefront of the block)
+    extern __shared__ int32_t s_dyn[];
+    int32_t *const s_slot = s_dyn + uniform(threadIdx.x / kWave) * (2 * kSmallChunk);
+    int32_t *const s_row = s_slot + kSmallChunk;
     if (*a.total > a.capacity) return;
-    const CellRange cr = my_cells(a.misc[kMiscCells]);
+    const CellRange cr = my_cells(a.misc[kMiscCells], a.cell_run);
     if (cr.first >= cr.end) return;
-    for (uint32_t cid = cr.first; cid < cr.end; cid += cr.step) {
+    for (uint32_t cid = cr.first, k = 1; cid < cr.end; cid = cr.next(cid, k)) {
         const CellLoad cur = load_cell(a, cid);
         const uint32_t state = uniform(cur.rec.state);
         if (state == kCellFast) {
@@ -841,7 +861,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
             const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
             const CellView v = make_view(a, cur, noff);
             if (v.total <= kSmallChunk)  // bigger cells: k_group_big
-                fill_small<KIND>(a, v, noff, s_slot[threadIdx.x / kWave], s_row[uniform(threadIdx.x / kWave)]);
+                fill_small<KIND>(a, v, noff, s_slot, s_row);
         }
     }
 }
@@ -872,18 +892,18 @@ __global__ void __launch_bounds__(kGroupThreads) k_group_big(GroupArgs a, Generi
 
 // Workgroups of `kernel` that the current device holds at once (all CUs),
 // cached per kernel and device.
-static int64_t resident_workgroups(const void *kernel) {
+static int64_t resident_workgroups(const void *kernel, int threads, size_t lds) {
     static std::mutex mu;
-    static std::map<std::pair<const void *, int>, int64_t> cache;
+    static std::map<std::tuple<const void *, int, int>, int64_t> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     std::lock_guard<std::mutex> lock(mu);
-    const auto key = std::make_pair(kernel, dev);
+    const auto key = std::make_tuple(kernel, dev, threads);
     const auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int per_cu = 0, cus = 0;
     int64_t r = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kGroupThreads, 0) == hipSuccess &&
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
         r = (int64_t)per_cu * cus;
     cache[key] = r;
@@ -896,12 +916,22 @@ static int64_t resident_workgroups(const void *kernel) {
 // kernel's tail at low occupancy), and no more than one wave per reference
 // point (a bound on the occupied cells); a multiple of the 8 XCDs (my_cells splits the cells by
 // blockIdx % 8).
-static unsigned group_grid(const Batch &b, const void *kernel) {
+// Threads per workgroup of the wavefront-per-cell kernels (the tuned block
+// size: 64, 128 or 256) and their dynamic LDS (words per wavefront).
+static int cell_threads() {
+    const int t = tuning().cell_threads;
+    return (t == 64 || t == 128) ? t : kGroupThreads;
+}
+static size_t cell_lds(int threads, int words_per_wave) {
+    return (size_t)(threads / kWave) * (size_t)words_per_wave * sizeof(int32_t);
+}
+
+static unsigned group_grid(const Batch &b, const void *kernel, int threads, size_t lds) {
     int64_t wgs = tuning().cell_workgroups;
     if (wgs <= 0) {
-        const int64_t resident = resident_workgroups(kernel);
+        const int64_t resident = resident_workgroups(kernel, threads, lds);
         wgs = resident > 0 ? (resident / 8) * 8 : 256 * 8;
-        wgs = std::min<int64_t>(wgs, (b.nr_total + 3) / 4);
+        wgs = std::min<int64_t>(wgs, (b.nr_total * (kGroupThreads / threads) + 3) / 4);
     }
     wgs = ((wgs + 7) / 8) * 8;
     return (unsigned)(wgs > 0 ? wgs : 8);
@@ -919,12 +949,16 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
     // 1024 workgroups: the stray list's wavefront-per-query pass wants more
     // waves than the big cells
     const dim3 block(kGroupThreads), bgrid(256), lgrid(1024);
+    const int ct = cell_threads();
+    const dim3 cblock(ct);
+    const size_t count_lds = cell_lds(ct, kSmallChunk), fill_lds = cell_lds(ct, 2 * kSmallChunk);
 #define TCR_L(D, M)                                                                                         \
     if (seen[D][M]) {                                                                                       \
         if (!FILL) {                                                                                        \
             kernel_begin(count_events(), s);                                                                \
-            hipLaunchKernelGGL((k_count_small<D, M>), dim3(group_grid(b, (const void *)k_count_small<D, M>)), \
-                               block, 0, s, ga);                                                            \
+            hipLaunchKernelGGL((k_count_small<D, M>),                                                       \
+                               dim3(group_grid(b, (const void *)k_count_small<D, M>, ct, count_lds)), cblock, \
+                               count_lds, s, ga);                                                           \
             kernel_end(count_events(), s);                                                                  \
         }                                                                                                   \
         if (last == D * 4 + M)                                                                              \
@@ -938,10 +972,10 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
     } else {
         fill_kernel_begin(s);
         switch (pair_kind(ga.out)) {
-            case 0: hipLaunchKernelGGL(k_fill_small<0>, dim3(group_grid(b, (const void *)k_fill_small<0>)), block, 0, s, ga); break;
-            case 1: hipLaunchKernelGGL(k_fill_small<1>, dim3(group_grid(b, (const void *)k_fill_small<1>)), block, 0, s, ga); break;
-            case 2: hipLaunchKernelGGL(k_fill_small<2>, dim3(group_grid(b, (const void *)k_fill_small<2>)), block, 0, s, ga); break;
-            default: hipLaunchKernelGGL(k_fill_small<3>, dim3(group_grid(b, (const void *)k_fill_small<3>)), block, 0, s, ga); break;
+            case 0: hipLaunchKernelGGL(k_fill_small<0>, dim3(group_grid(b, (const void *)k_fill_small<0>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            case 1: hipLaunchKernelGGL(k_fill_small<1>, dim3(group_grid(b, (const void *)k_fill_small<1>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            case 2: hipLaunchKernelGGL(k_fill_small<2>, dim3(group_grid(b, (const void *)k_fill_small<2>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            default: hipLaunchKernelGGL(k_fill_small<3>, dim3(group_grid(b, (const void *)k_fill_small<3>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
         }
         fill_kernel_end(s);
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
@@ -982,6 +1016,7 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
     ga.stray_count = w.misc + kMiscStrays;
     ga.big_list = w.big_list;
     ga.big_count = w.misc + kMiscBig;
+    ga.cell_run = (uint32_t)std::max(1, tuning().cells_per_wave);
     return ga;
 }
 

@@ -199,6 +199,8 @@ int csr_arrays_launch(const int64_t *offsets, int64_t n, int32_t *indptr, int64_
 struct Tuning {
     int32_t cell_workgroups = 0;
     int32_t stencil_workgroups = 0;
+    int32_t cell_threads = 0;     // 0: 256
+    int32_t cells_per_wave = 0;   // 0: 1
 };
 Tuning &tuning();
 

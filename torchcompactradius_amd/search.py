@@ -213,7 +213,7 @@ class SearchPlan:
         self.key = (self.n, self.nq, self.nr, tuple((c.dim, c.mode) for c in self.clouds[:8]))
         self._graph = None
         self._graph_out = None
-        self.tuning = None          # (cell_workgroups, stencil_workgroups) or None: automatic
+        self.tuning = None          # launch shape (see autotune.py) or None: automatic
 
     def _install_tuning(self):
         from .autotune import install
