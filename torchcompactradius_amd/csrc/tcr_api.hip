@@ -45,6 +45,9 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err, 
         d.rt_off = b.rt_total;
         d.rt_n = tiles_of(d.h.nr);
         b.rt_total += d.rt_n;
+        d.qt_off = b.qt_total;
+        d.qt_n = takes_query_cells(d.h) ? tiles_of(h.nq) : 0;
+        b.qt_total += d.qt_n;
         d.hb_off = b.hb_total;
         b.hb_total += h.hash_map_length;
         d.lt_off = b.lt_total;
@@ -66,19 +69,20 @@ bool make_batch(const tcr_cloud *clouds, int32_t n, Batch &b, std::string &err, 
     return true;
 }
 
-// Bytes of the header block: the descriptors, the five prefix arrays
-// (q_start, r_start, rt_start, hb_start, lt_start) and the zeroed device
-// counters, contiguous so that one upload writes them all.
+// Bytes of the header block: the descriptors, the six prefix arrays
+// (q_start, r_start, rt_start, hb_start, lt_start, qt_start) and the zeroed
+// device counters, contiguous so that one upload writes them all.
+constexpr int kPrefixes = 6;
 static size_t header_bytes(int64_t B) {
-    return sizeof(CloudDev) * B + 5 * sizeof(int64_t) * (B + 1) + sizeof(uint32_t) * kMiscWords;
+    return sizeof(CloudDev) * B + kPrefixes * sizeof(int64_t) * (B + 1) + sizeof(uint32_t) * kMiscWords;
 }
 
 static void header_views(char *base, int64_t B, Work &w) {
     w.desc = (CloudDev *)base;
     int64_t *pre = base ? (int64_t *)(base + sizeof(CloudDev) * B) : nullptr;
-    int64_t **dst[5] = {&w.q_start, &w.r_start, &w.rt_start, &w.hb_start, &w.lt_start};
-    for (int k = 0; k < 5; ++k) *dst[k] = pre ? pre + k * (B + 1) : nullptr;
-    w.misc = pre ? (uint32_t *)(pre + 5 * (B + 1)) : nullptr;
+    int64_t **dst[kPrefixes] = {&w.q_start, &w.r_start, &w.rt_start, &w.hb_start, &w.lt_start, &w.qt_start};
+    for (int k = 0; k < kPrefixes; ++k) *dst[k] = pre ? pre + k * (B + 1) : nullptr;
+    w.misc = pre ? (uint32_t *)(pre + kPrefixes * (B + 1)) : nullptr;
 }
 
 // Upload of the header through kernel arguments: no host staging buffer to
@@ -142,6 +146,29 @@ size_t carve_work(const Batch &b, void *ws, Work &w) {
     w.ranges = cv.take<int2>(27 * (size_t)nr);
     w.recs = cv.take<float4>(2 * (size_t)nr);
     w.big_list = cv.take<uint32_t>(nr);
+    // query cells (only when some cloud takes them)
+    const int64_t nqc = b.qt_total > 0 ? nq : 0;
+    w.qwrap = cv.take<float4>(nqc);
+    w.qkeys0 = cv.take<uint32_t>(nqc);
+    w.qkeys1 = cv.take<uint32_t>(nqc);
+    w.qvals0 = cv.take<uint32_t>(nqc);
+    w.qvals1 = cv.take<uint32_t>(nqc);
+    w.qpflags = cv.take<uint8_t>(nqc);
+    w.qsort_hist = cv.take<uint32_t>(nqc ? (size_t)B * kMaxPasses * kRadix : 0);
+    w.qsort_lb = cv.take<uint64_t>((size_t)kMaxPasses * b.qt_total * kRadix);
+    w.qcell_lb = cv.take<uint64_t>(b.qt_total);
+    w.qsorted = cv.take<float4>(nqc);
+    w.qsort_index = cv.take<int32_t>(nqc);
+    w.qskey = cv.take<int32_t>(nqc);
+    w.qsflags = cv.take<uint8_t>(nqc);
+    w.qcell_begin = cv.take<uint32_t>(nqc ? nqc + 1 : 0);
+    w.qcell_lin = cv.take<int32_t>(nqc);
+    w.qcell_cloud = cv.take<uint32_t>(nqc);
+    w.qmasks = cv.take<uint64_t>(7 * (size_t)nqc);
+    w.qranges = cv.take<int2>(27 * (size_t)nqc);
+    w.qrecs = cv.take<float4>(2 * (size_t)nqc);
+    w.qbig_list = cv.take<uint32_t>(nqc);
+    w.qstray_list = cv.take<int32_t>(nqc);
     return cv.off + 256;
 }
 
@@ -169,7 +196,8 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
     p.header.assign(header_bytes(b.B), 0);
     Work hv;
     header_views(p.header.data(), b.B, hv);
-    int64_t *qs = hv.q_start, *rs = hv.r_start, *rts = hv.rt_start, *hbs = hv.hb_start, *lts = hv.lt_start;
+    int64_t *qs = hv.q_start, *rs = hv.r_start, *rts = hv.rt_start, *hbs = hv.hb_start, *lts = hv.lt_start,
+            *qts = hv.qt_start;
     for (int32_t c = 0; c < b.B; ++c) {
         CloudDev &d = b.dev[c];
         const tcr_cloud &h = clouds[c];
@@ -192,6 +220,7 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
         rts[c + 1] = rts[c] + d.rt_n;
         hbs[c + 1] = hbs[c] + h.hash_map_length;
         lts[c + 1] = lts[c] + d.lt_cap;
+        qts[c + 1] = qts[c] + d.qt_n;
     }
     if (upload) {
         std::memcpy(hv.desc, b.dev.data(), sizeof(CloudDev) * b.B);

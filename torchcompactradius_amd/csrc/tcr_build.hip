@@ -55,14 +55,15 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
     for (int d = 1; d < dim; ++d) hc = (h[d] > hc || isnan(h[d])) ? h[d] : hc;
     cd.hcell = hc;
     uint32_t err = 0;
-    // wavefront-per-cell eligibility: self-search, stencil radius 1, a
-    // finite positive cell size, and >= 5 cells in every periodic dimension.
+    // wavefront-per-cell eligibility: stencil radius 1, a finite positive
+    // cell size, and >= 5 cells in every periodic dimension (the cells are
+    // the reference set's for a self-search, the query cells otherwise).
     // A pair within support (<= hCell) has its minimal image within 1.5 cells
     // of the query's cell centre, and nc = ceil(ext / hCell) >= 5 gives
     // ext > 4 cells: the image nearest the centre (|offset| <= ext / 2 > 2
     // cells) is that minimal image, with half a cell to spare for rounding,
     // and the 3 stencil cells of a dimension are distinct.
-    uint32_t fast = (cd.h.self_search && cd.h.search_range == 1 && hc > 0.0f && !isinf(hc)) ? 1u : 0u;
+    uint32_t fast = (cd.h.search_range == 1 && hc > 0.0f && !isinf(hc)) ? 1u : 0u;
     for (int d = 0; d < 3; ++d) {
         if (d < dim) {
             const float v = ceilf(ext[d] / hc);
@@ -72,6 +73,13 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
         } else {
             cd.nc[d] = 1;
         }
+    }
+    // query cells decode their coordinates from the linear index: a grid of
+    // more than 2^31 cells (whose index wraps) leaves them to the per-query path
+    if (!cd.h.self_search && fast) {
+        uint64_t cells = 1;
+        for (int d = 0; d < dim; ++d) cells *= (uint64_t)cd.nc[d];
+        if (cells >= (1ull << 31)) fast = 0;
     }
     cd.err = err;
     cd.fast = fast;
@@ -419,6 +427,14 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
         clear.p[2] = (uint32_t *)w.cell_lb;
         clear.n[2] = 2 * b.rt_total;
     }
+    if (b.qt_total > 0) {
+        clear.p[3] = w.qsort_hist;
+        clear.n[3] = (int64_t)b.B * kMaxPasses * kRadix;
+        clear.p[4] = (uint32_t *)w.qsort_lb;
+        clear.n[4] = 2 * (int64_t)kMaxPasses * b.qt_total * kRadix;
+        clear.p[5] = (uint32_t *)w.qcell_lb;
+        clear.n[5] = 2 * b.qt_total;
+    }
     hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, b.B,
                        (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
                        clear);
@@ -474,7 +490,172 @@ int build_structure(const Batch &b, Work &w, hipStream_t s) {
                        (const int64_t *)w.rt_start, (const int64_t *)w.r_start, w.rwrap, w.keys0, w.vals0,
                        w.pflags, w.sort_hist, w.misc);
     TCR_LAUNCH_CHECK();
-    return build_sorted_cells(b, w, s, true);
+    rc = build_sorted_cells(b, w, s, true);
+    if (rc) return rc;
+    return build_queries(b, w, s);
+}
+
+// ------------------------------------------------------------ query cells --
+
+// A query of a query-cell cloud: its once-wrapped position (the query set is
+// wrapped once, neighborhood.py:202), the linear index of its cell in the
+// reference grid as key, and bit 1 when that cell lies outside the grid
+// (key 0 then: such a query goes to the per-query path, whatever cell it
+// sorts into).
+template <int DIM>
+__device__ __forceinline__ void query_key_one(const CloudDev &cd, int64_t local, float4 &wp, int32_t &lin,
+                                              uint8_t &flags) {
+    const float *p = cd.qp + local * DIM;
+    float x[3] = {0.f, 0.f, 0.f};
+    int32_t c[3] = {0, 0, 0};
+    bool out = false;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        float v = p[d];
+        if ((cd.h.periodic_mask >> d) & 1u)
+            v = wrap_coord(v, cd.h.domain_min[d], cd.h.domain_max[d] - cd.h.domain_min[d]);
+        x[d] = v;
+        c[d] = cell_coord(v, cd.h.domain_min[d], cd.hcell);
+        if (c[d] < 0 || c[d] >= cd.nc[d]) out = true;
+    }
+    lin = out ? 0 : linear_index<DIM>(c, cd.nc);
+    wp = make_float4(x[0], x[1], x[2], cd.qs ? cd.qs[local] : 0.0f);
+    flags = out ? 2 : 0;
+}
+
+// One sort tile of queries per workgroup: keys, wrapped positions, flags and
+// the tile's digit counts of every radix pass (as k_keys for the reference
+// points); the query sort runs the passes the grid's cell count needs.
+__global__ void __launch_bounds__(kSortThreads) k_qkeys(CloudDev *__restrict__ desc, int32_t B,
+                                                        const int64_t *__restrict__ t_start,
+                                                        const int64_t *__restrict__ p_start,
+                                                        float4 *__restrict__ qwrap, uint32_t *__restrict__ keys,
+                                                        uint32_t *__restrict__ vals, uint8_t *__restrict__ pflags,
+                                                        uint32_t *__restrict__ hist, uint32_t *__restrict__ misc) {
+    __shared__ uint32_t h[kMaxPasses][kRadix];
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) (&h[0][0])[k] = 0;
+    __syncthreads();
+    const TileInfo ti = tile_info(blockIdx.x, B, t_start, p_start);
+    const CloudDev &cd = desc[ti.c];
+    const int64_t cstart = p_start[ti.c];
+    const int dim = cd.h.dim;
+    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
+        const int64_t g = ti.start + k;
+        const int64_t local = g - cstart;
+        float4 wp;
+        int32_t lin;
+        uint8_t f;
+        switch (dim) {
+            case 1: query_key_one<1>(cd, local, wp, lin, f); break;
+            case 2: query_key_one<2>(cd, local, wp, lin, f); break;
+            default: query_key_one<3>(cd, local, wp, lin, f); break;
+        }
+        qwrap[g] = wp;
+        keys[g] = (uint32_t)lin;
+        vals[g] = (uint32_t)local;
+        pflags[g] = f;
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p)
+            atomicAdd(&h[p][((uint32_t)lin >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;
+    for (int k = threadIdx.x; k < kMaxPasses * kRadix; k += kSortThreads) {
+        const uint32_t v = (&h[0][0])[k];
+        if (v) atomicAdd(&hc[k], v);
+    }
+    if (threadIdx.x == 0) atomicMax(&misc[kMiscQPasses], cloud_passes(cd));
+}
+
+// Query cells: runs of equal keys in the sorted queries, numbered across the
+// batch by decoupled look-back over the query tiles (as k_cells numbers the
+// reference cells; no cell index is built for them).
+__global__ void __launch_bounds__(kSortThreads) k_qcells(const int64_t *__restrict__ t_start,
+                                                         const int64_t *__restrict__ p_start, int32_t B,
+                                                         int64_t ntiles, const int32_t *__restrict__ skey,
+                                                         uint32_t *__restrict__ cell_begin,
+                                                         int32_t *__restrict__ cell_lin,
+                                                         uint32_t *__restrict__ cell_cloud, uint64_t *__restrict__ lb,
+                                                         uint32_t *__restrict__ misc, int64_t nq) {
+    constexpr int kItems = kSortTile / kSortThreads;
+    __shared__ uint32_t s_wsum[kSortThreads / kWave];
+    __shared__ int64_t s_tile;
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&misc[kMiscQCellTicket], 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int c = find_segment(B, tile, [&](int k) { return t_start[k]; });
+    const int64_t first_tile = t_start[c];
+    const int64_t cstart = p_start[c], cn = p_start[c + 1] - cstart;
+    const int64_t t0 = cstart + (tile - first_tile) * kSortTile;
+    const int64_t len = min((int64_t)kSortTile, cn - (tile - first_tile) * kSortTile);
+    const int64_t s0 = t0 + (int64_t)threadIdx.x * kItems;
+    uint32_t fl = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t s = s0 + k;
+        if (s - t0 < len) {
+            const bool f = (s == cstart) || (skey[s] != skey[s - 1]);
+            fl |= (uint32_t)f << k;
+            cnt += f;
+        }
+    }
+    uint32_t incl = cnt;
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += v;
+    }
+    if (lane == kWave - 1) s_wsum[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    uint32_t wpre = 0, ttot = 0;
+    for (int w = 0; w < kSortThreads / kWave; ++w) {
+        if (w < (int)threadIdx.x / kWave) wpre += s_wsum[w];
+        ttot += s_wsum[w];
+    }
+    if (threadIdx.x == 0) {
+        uint64_t *l = lb + tile;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            lb_store(l, kLbInc | ttot);
+        } else {
+            lb_store(l, kLbAgg | ttot);
+            excl = look_back(lb, 1, tile - 1, 0);
+            lb_store(l, kLbInc | (excl + ttot));
+        }
+        s_excl = excl;
+        if (tile == ntiles - 1) {
+            misc[kMiscQCells] = (uint32_t)(excl + ttot);
+            cell_begin[excl + ttot] = (uint32_t)nq;
+        }
+    }
+    __syncthreads();
+    uint32_t cid = (uint32_t)(s_excl + wpre + incl - cnt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (!((fl >> k) & 1u)) continue;
+        const int64_t s = s0 + k;
+        cell_begin[cid] = (uint32_t)s;
+        cell_lin[cid] = skey[s];
+        cell_cloud[cid] = (uint32_t)c;
+        ++cid;
+    }
+}
+
+int build_queries(const Batch &b, Work &w, hipStream_t s) {
+    if (b.qt_total == 0) return TCR_OK;
+    hipLaunchKernelGGL(k_qkeys, dim3((unsigned)b.qt_total), dim3(kSortThreads), 0, s, w.desc, b.B,
+                       (const int64_t *)w.qt_start, (const int64_t *)w.q_start, w.qwrap, w.qkeys0, w.qvals0,
+                       w.qpflags, w.qsort_hist, w.misc);
+    TCR_LAUNCH_CHECK();
+    int rc = radix_sort_queries(b, w, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_qcells, dim3((unsigned)b.qt_total), dim3(kSortThreads), 0, s, (const int64_t *)w.qt_start,
+                       (const int64_t *)w.q_start, b.B, b.qt_total, (const int32_t *)w.qskey, w.qcell_begin,
+                       w.qcell_lin, w.qcell_cloud, w.qcell_lb, w.misc, b.nq_total);
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
 }
 
 // ------------------------------------------ reference-layout bucket tables --

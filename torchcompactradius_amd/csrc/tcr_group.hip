@@ -144,6 +144,11 @@ struct GroupArgs {
     const float4 *sorted;
     const int32_t *sort_index;   // local original index per sorted slot
     const uint8_t *sflags;       // per sorted slot: nonzero = stray query
+    // the member queries of a cell: the sorted reference points for a
+    // self-search, the sorted queries of a query-cell cloud
+    const float4 *msorted;
+    const int32_t *msort_index;
+    const uint8_t *msflags;
     int2 *ranges;                // [cells * 27] (first global slot, count)
     CellRec *recs;               // [cells]
     uint64_t *masks;             // [kSmallRounds][nr]: round-major, so a batch's loads and stores coalesce
@@ -159,6 +164,11 @@ struct GroupArgs {
     uint32_t *big_list;          // [cells]
     uint32_t *big_count;
     uint32_t cell_run;           // consecutive cells a wavefront takes in turn (tuned)
+    const uint32_t *ncells;      // the cell count of this pass (misc[kMiscCells] or misc[kMiscQCells])
+    // query cells (k_qstencil)
+    const uint32_t *qcell_begin;
+    const int32_t *qcell_lin;
+    const uint32_t *qcell_cloud;
 };
 
 // ------------------------------------------------------------ stencil ----
@@ -279,6 +289,47 @@ __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
     }
 }
 
+// One thread per query cell of the query-cell clouds: its record (query
+// slots, cloud, state, centre) and its resolved stencil in the reference
+// structure.  The cell's coordinates come from its key (the linear index of
+// an in-grid cell); its queries outside the grid carry a stray flag and go
+// to the per-query path one by one.  A cloud the cell kernels do not take
+// (fewer than 5 cells in a periodic dimension, a degenerate grid) is left to
+// the per-query kernel whole.
+__global__ void __launch_bounds__(256) k_qstencil(GroupArgs a) {
+    const uint32_t ncells = *a.ncells;
+    for (int64_t cid = (int64_t)blockIdx.x * 256 + threadIdx.x; cid < ncells; cid += (int64_t)gridDim.x * 256) {
+        const uint32_t c = a.qcell_cloud[cid];
+        const CloudDev &cd = a.desc[c];
+        const int dim = cd.h.dim;
+        // a cloud's last cell ends with its queries (the next cloud's may
+        // not be sorted into cells)
+        const uint32_t gb = a.qcell_begin[cid];
+        const uint32_t ge = (uint32_t)min((int64_t)a.qcell_begin[cid + 1], a.q_start[c + 1]);
+        int2 *out = a.ranges + cid * kStencil;
+        if (!cd.fast) {
+            a.recs[cid] = CellRec{gb, ge, c, kCellSkip, make_float4(0, 0, 0, 0)};
+            continue;
+        }
+        int32_t q[3] = {0, 0, 0};
+        uint32_t lin = (uint32_t)a.qcell_lin[cid];
+        float cen[3] = {0.f, 0.f, 0.f};
+        bool wrap = false;
+        for (int d = 0; d < dim; ++d) {
+            q[d] = (int32_t)(lin % (uint32_t)cd.nc[d]);
+            lin /= (uint32_t)cd.nc[d];
+            cen[d] = cd.h.domain_min[d] + ((float)q[d] + 0.5f) * cd.hcell;
+            if (((cd.h.periodic_mask >> d) & 1u) && (q[d] == 0 || q[d] == cd.nc[d] - 1)) wrap = true;
+        }
+        a.recs[cid] = CellRec{gb, ge, c, kCellFast, make_float4(cen[0], cen[1], cen[2], wrap ? 1.0f : 0.0f)};
+        switch (dim) {
+            case 1: stencil_cell<1>(a, cd, q, out); break;
+            case 2: stencil_cell<2>(a, cd, q, out); break;
+            default: stencil_cell<3>(a, cd, q, out); break;
+        }
+    }
+}
+
 // ------------------------------------------------------- per-cell setup ---
 
 // One cell as a wavefront sees it: the record, and lane o holding stencil
@@ -357,7 +408,7 @@ __device__ __forceinline__ void expand_slots(const CellView &v, int32_t *slot_of
 __device__ __forceinline__ void append_strays(const GroupArgs &a, const CellView &v) {
     for (uint32_t s = v.gb + __lane_id(); s < v.ge; s += kWave) {
         const uint32_t k = atomicAdd(a.stray_count, 1u);
-        a.stray_list[k] = (int32_t)(v.qs + a.sort_index[s]);
+        a.stray_list[k] = (int32_t)(v.qs + a.msort_index[s]);
     }
 }
 
@@ -366,9 +417,9 @@ __device__ __forceinline__ void append_strays(const GroupArgs &a, const CellView
 // the per-query list; the cell kernels skip them.
 __device__ __forceinline__ void append_flagged(const GroupArgs &a, const CellView &v) {
     for (uint32_t s = v.gb + __lane_id(); s < v.ge; s += kWave) {
-        if (a.sflags[s]) {
+        if (a.msflags[s]) {
             const uint32_t k = atomicAdd(a.stray_count, 1u);
-            a.stray_list[k] = (int32_t)(v.qs + a.sort_index[s]);
+            a.stray_list[k] = (int32_t)(v.qs + a.msort_index[s]);
         }
     }
 }
@@ -447,7 +498,7 @@ struct Geo {
 template <int DIM, int MODE>
 __device__ __attribute__((noinline)) bool exact_pair(const GroupArgs &a, const CloudDev *cd, uint32_t qs_slot,
                                                      int32_t j) {
-    const float4 xq = a.sorted[qs_slot];
+    const float4 xq = a.msorted[qs_slot];
     const float xi[3] = {xq.x, xq.y, xq.z};
     const float4 v = a.sorted[j];
     const float xj[3] = {v.x, v.y, v.z};
@@ -527,7 +578,7 @@ __device__ __forceinline__ void test_members(const GroupArgs &a, const CloudDev 
             const uint64_t hitm = round_hits<DIM, MODE>(
                 a, cd, g, cx[r], cp[r], chh[r], tj[r], xi2, pi, hi, tq, mb + m, [&]() {
                     const int32_t k = r * kWave + lane;
-                    return k < v.total ? k + slot_off[k] : (int32_t)v.gb;
+                    return k < v.total ? k + slot_off[k] : 0;
                 });
             mk_lo[r] = put_lane(mk_lo[r], (uint32_t)hitm, m);
             mk_hi[r] = put_lane(mk_hi[r], (uint32_t)(hitm >> 32), m);
@@ -543,9 +594,9 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
     // first member batch: independent of the candidates, issue it first
     const uint32_t nm0 = min((uint32_t)kWave, v.ge - v.gb);
     const uint32_t ms0 = (uint32_t)lane < nm0 ? v.gb + lane : v.gb;
-    uint32_t mfl = (uint32_t)lane < nm0 ? a.sflags[ms0] : 0u;
-    int32_t mloc = a.sort_index[ms0];
-    float4 mq = a.sorted[ms0];
+    uint32_t mfl = (uint32_t)lane < nm0 ? a.msflags[ms0] : 0u;
+    int32_t mloc = a.msort_index[ms0];
+    float4 mq = a.msorted[ms0];
 
     const Geo<DIM> g(cd, v.center);
     const int nrounds = (v.total + kWave - 1) / kWave;  // 1..kSmallRounds
@@ -562,7 +613,7 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
         if (r >= nrounds) continue;  // uniform
         const int32_t k = r * kWave + lane;
         const bool valid = k < v.total;
-        const int32_t j = valid ? k + slot_off[k] : (int32_t)v.gb;
+        const int32_t j = valid ? k + slot_off[k] : 0;
         const float4 cv = a.sorted[j];
         if (v.wrap) g.place(cv, valid, cx[r], cp[r]);
         else g.place_direct(cv, valid, cx[r], cp[r]);
@@ -575,9 +626,9 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
         const bool mvalid = (uint32_t)lane < nm;
         const uint32_t ms = mvalid ? mb + lane : v.gb;
         if (mb != v.gb) {
-            mfl = mvalid ? a.sflags[ms] : 0u;
-            mloc = a.sort_index[ms];
-            mq = a.sorted[ms];
+            mfl = mvalid ? a.msflags[ms] : 0u;
+            mloc = a.msort_index[ms];
+            mq = a.msorted[ms];
         }
         const int64_t mqg = v.qs + mloc;
         if (mvalid && mfl) {
@@ -653,12 +704,12 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
     // first member batch: independent of the candidates, issue it first
     const uint32_t nm0 = min((uint32_t)kWave, v.ge - v.gb);
     const uint32_t ms0 = (uint32_t)lane < nm0 ? v.gb + lane : v.gb;
-    bool live = (uint32_t)lane < nm0 && !a.sflags[ms0];
+    bool live = (uint32_t)lane < nm0 && !a.msflags[ms0];
     uint64_t mk[kSmallRounds];
 #pragma unroll
     for (int r = 0; r < kSmallRounds; ++r)
         mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms0] : 0ull;
-    int32_t mloc = a.sort_index[ms0];
+    int32_t mloc = a.msort_index[ms0];
     int64_t mbase = live ? a.offsets[v.qs + mloc] : 0;
 
     expand_slots(v, slot_off, noff);
@@ -673,11 +724,11 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
         const bool mvalid = (uint32_t)lane < nm;
         const uint32_t ms = mvalid ? mb + lane : v.gb;
         if (mb != v.gb) {
-            live = mvalid && !a.sflags[ms];
+            live = mvalid && !a.msflags[ms];
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
                 mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms] : 0ull;
-            mloc = a.sort_index[ms];
+            mloc = a.msort_index[ms];
             mbase = live ? a.offsets[v.qs + mloc] : 0;
         }
         const uint64_t todo = __ballot(live);
@@ -726,16 +777,16 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
             const bool valid = r < nrounds && k < v.total;
             const int32_t j = candidate_slot(v, k, kOff);
             corig[r] = (FILL && valid) ? a.sort_index[j] : 0;
-            const float4 cv = a.sorted[valid ? j : v.gb];
+            const float4 cv = a.sorted[valid ? j : 0];
             g.place(cv, valid, cx[r], cp[r]);
             chh[r] = cv.w;
             if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
         }
         for (uint32_t s = v.gb; s < v.ge; ++s) {
-            if (a.sflags[s]) continue;  // listed for the per-query kernel by k_count_small
-            const int32_t qloc = uniform(a.sort_index[s]);
+            if (a.msflags[s]) continue;  // listed for the per-query kernel by k_count_small
+            const int32_t qloc = uniform(a.msort_index[s]);
             const int64_t qg = v.qs + qloc;
-            const float4 mq = a.sorted[s];
+            const float4 mq = a.msorted[s];
             const float mqv[3] = {mq.x, mq.y, mq.z};
             float xi2[3] = {0.f, 0.f, 0.f}, pi = 0.0f;
 #pragma unroll
@@ -823,7 +874,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     // wavefront of the block)
     extern __shared__ int32_t s_dyn[];
     int32_t *const s_slot = s_dyn + uniform(threadIdx.x / kWave) * kSmallChunk;
-    const CellRange cr = my_cells(a.misc[kMiscCells], a.cell_run);
+    const CellRange cr = my_cells(*a.ncells, a.cell_run);
     if (cr.first >= cr.end) return;
     // no next-cell prefetch here: its registers would cost the occupancy
     // that hides the same latency
@@ -851,7 +902,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     int32_t *const s_slot = s_dyn + uniform(threadIdx.x / kWave) * (2 * kSmallChunk);
     int32_t *const s_row = s_slot + kSmallChunk;
     if (*a.total > a.capacity) return;
-    const CellRange cr = my_cells(a.misc[kMiscCells], a.cell_run);
+    const CellRange cr = my_cells(*a.ncells, a.cell_run);
     if (cr.first >= cr.end) return;
     for (uint32_t cid = cr.first, k = 1; cid < cr.end; cid = cr.next(cid, k)) {
         const CellLoad cur = load_cell(a, cid);
@@ -926,23 +977,26 @@ static size_t cell_lds(int threads, int words_per_wave) {
     return (size_t)(threads / kWave) * (size_t)words_per_wave * sizeof(int32_t);
 }
 
-static unsigned group_grid(const Batch &b, const void *kernel, int threads, size_t lds) {
+static unsigned group_grid(int64_t npts, const void *kernel, int threads, size_t lds) {
     int64_t wgs = tuning().cell_workgroups;
     if (wgs <= 0) {
         const int64_t resident = resident_workgroups(kernel, threads, lds);
         wgs = resident > 0 ? (resident / 8) * 8 : 256 * 8;
-        wgs = std::min<int64_t>(wgs, (b.nr_total * (kGroupThreads / threads) + 3) / 4);
+        wgs = std::min<int64_t>(wgs, (npts * (kGroupThreads / threads) + 3) / 4);
     }
     wgs = ((wgs + 7) / 8) * 8;
     return (unsigned)(wgs > 0 ? wgs : 8);
 }
 
 template <bool FILL>
-static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &gen, hipStream_t s) {
+static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &gen, hipStream_t s, bool qcells) {
     bool seen[4][4] = {};
     int last = -1;  // (dim, mode) of the last big-cell launch, which also takes the stray list
-    for (const CloudDev &d : b.dev)
-        if (d.h.self_search && d.h.search_range == 1 && d.h.nq > 0) seen[d.h.dim][d.h.mode] = true;
+    for (const CloudDev &d : b.dev) {
+        const bool mine = qcells ? takes_query_cells(d.h) : (d.h.self_search && d.h.search_range == 1 && d.h.nq > 0);
+        if (mine) seen[d.h.dim][d.h.mode] = true;
+    }
+    const int64_t npts = qcells ? b.nq_total : b.nr_total;
     for (int d = 1; d <= 3; ++d)
         for (int m = 0; m < 4; ++m)
             if (seen[d][m]) last = d * 4 + m;
@@ -957,7 +1011,7 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
         if (!FILL) {                                                                                        \
             kernel_begin(count_events(), s);                                                                \
             hipLaunchKernelGGL((k_count_small<D, M>),                                                       \
-                               dim3(group_grid(b, (const void *)k_count_small<D, M>, ct, count_lds)), cblock, \
+                               dim3(group_grid(npts, (const void *)k_count_small<D, M>, ct, count_lds)), cblock, \
                                count_lds, s, ga);                                                           \
             kernel_end(count_events(), s);                                                                  \
         }                                                                                                   \
@@ -972,10 +1026,10 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
     } else {
         fill_kernel_begin(s);
         switch (pair_kind(ga.out)) {
-            case 0: hipLaunchKernelGGL(k_fill_small<0>, dim3(group_grid(b, (const void *)k_fill_small<0>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
-            case 1: hipLaunchKernelGGL(k_fill_small<1>, dim3(group_grid(b, (const void *)k_fill_small<1>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
-            case 2: hipLaunchKernelGGL(k_fill_small<2>, dim3(group_grid(b, (const void *)k_fill_small<2>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
-            default: hipLaunchKernelGGL(k_fill_small<3>, dim3(group_grid(b, (const void *)k_fill_small<3>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            case 0: hipLaunchKernelGGL(k_fill_small<0>, dim3(group_grid(npts, (const void *)k_fill_small<0>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            case 1: hipLaunchKernelGGL(k_fill_small<1>, dim3(group_grid(npts, (const void *)k_fill_small<1>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            case 2: hipLaunchKernelGGL(k_fill_small<2>, dim3(group_grid(npts, (const void *)k_fill_small<2>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
+            default: hipLaunchKernelGGL(k_fill_small<3>, dim3(group_grid(npts, (const void *)k_fill_small<3>, ct, fill_lds)), cblock, fill_lds, s, ga); break;
         }
         fill_kernel_end(s);
         TCR_LD(1) TCR_LD(2) TCR_LD(3)
@@ -1017,6 +1071,34 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
     ga.big_list = w.big_list;
     ga.big_count = w.misc + kMiscBig;
     ga.cell_run = (uint32_t)std::max(1, tuning().cells_per_wave);
+    ga.ncells = w.misc + kMiscCells;
+    ga.msorted = w.sorted;
+    ga.msort_index = w.sort_index;
+    ga.msflags = w.sflags;
+    ga.qcell_begin = w.qcell_begin;
+    ga.qcell_lin = w.qcell_lin;
+    ga.qcell_cloud = w.qcell_cloud;
+    return ga;
+}
+
+// The same for the query cells: members are the sorted queries; the cell
+// records, stencil ranges, masks and lists are the query cells' own; the
+// candidate lookups stay in the reference structure.
+static GroupArgs query_cell_args(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity) {
+    GroupArgs ga = group_args(b, w, offsets, out, capacity);
+    ga.cell_cloud = w.qcell_cloud;   // cloud of a (query) cell id, for the big-cell kernel
+    ga.ranges = w.qranges;
+    ga.recs = (CellRec *)w.qrecs;
+    ga.masks = w.qmasks;
+    ga.mask_stride = b.nq_total;
+    ga.stray_list = w.qstray_list;
+    ga.stray_count = w.misc + kMiscQStrays;
+    ga.big_list = w.qbig_list;
+    ga.big_count = w.misc + kMiscQBig;
+    ga.ncells = w.misc + kMiscQCells;
+    ga.msorted = w.qsorted;
+    ga.msort_index = w.qsort_index;
+    ga.msflags = w.qsflags;
     return ga;
 }
 
@@ -1025,10 +1107,11 @@ static GroupArgs group_args(const Batch &b, Work &w, const int64_t *offsets, Pai
 // per-query kernel for every other cloud and for the strays.
 static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut out, int64_t capacity, bool fill,
                       hipStream_t s) {
-    bool any_group = false, any_other = false;
+    bool any_group = false, any_qcells = false, any_other = false;
     for (const CloudDev &d : b.dev) {
         if (d.h.nq == 0) continue;
         if (d.h.self_search && d.h.search_range == 1) any_group = true;
+        else if (takes_query_cells(d.h)) any_qcells = true;
         else any_other = true;
     }
     if (b.nq_total == 0) return TCR_OK;
@@ -1044,10 +1127,25 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
             hipLaunchKernelGGL(k_stencil, dim3(sgrid), dim3(256), 0, s, ga);
         }
         const GenericArgs gen = generic_args(b, w, offsets, out, capacity);
-        const int rc = fill ? launch_modes<true>(b, ga, gen, s) : launch_modes<false>(b, ga, gen, s);
+        const int rc = fill ? launch_modes<true>(b, ga, gen, s, false) : launch_modes<false>(b, ga, gen, s, false);
         if (rc) return rc;
     }
-    if (any_other) return launch_generic(b, w, offsets, out, capacity, fill, s);
+    if (any_qcells) {
+        const GroupArgs qa = query_cell_args(b, w, offsets, out, capacity);
+        if (!fill) {
+            // one thread per query cell (at most one per query)
+            const int32_t sw = tuning().stencil_workgroups;
+            const unsigned sgrid = sw > 0 ? (unsigned)sw : (unsigned)std::max<int64_t>(1, (b.nq_total + 255) / 256);
+            hipLaunchKernelGGL(k_qstencil, dim3(sgrid), dim3(256), 0, s, qa);
+        }
+        GenericArgs gen = generic_args(b, w, offsets, out, capacity);
+        gen.list = w.qstray_list;
+        gen.list_count = w.misc + kMiscQStrays;
+        const int rc = fill ? launch_modes<true>(b, qa, gen, s, true) : launch_modes<false>(b, qa, gen, s, true);
+        if (rc) return rc;
+    }
+    // every other cloud, and query-cell clouds the cell kernels do not take
+    if (any_other || any_qcells) return launch_generic(b, w, offsets, out, capacity, fill, s);
     return TCR_OK;
 }
 

@@ -32,7 +32,14 @@ enum : int {
     kMiscMlmFlags = 7,   // multi-level search: inputs it does not handle (see tcr_mlm.hip)
     kMiscSortTicket = 8, // tile tickets of the radix passes (kMaxPasses)
     kMiscLevelMax = 12,  // multi-level search: ordered max support per level (4 words)
-    kMiscWords = 16,
+    // the query cells of query != reference clouds (see build_queries)
+    kMiscQCells = 16,      // occupied query cells of the batch
+    kMiscQPasses = 17,     // radix passes of the query sort
+    kMiscQCellTicket = 18, // tile ticket of k_qcells
+    kMiscQStrays = 19,     // queries the query-cell count pass leaves to the per-query path
+    kMiscQBig = 20,        // query cells with more candidates than one register chunk
+    kMiscQSortTicket = 24, // tile tickets of the query radix passes (kMaxPasses)
+    kMiscWords = 32,
 };
 
 // Error reporting (thread-local message; see tcr_last_error()).
@@ -74,6 +81,7 @@ struct Batch {
     int32_t B = 0;
     int64_t nq_total = 0, nr_total = 0;   // points
     int64_t rt_total = 0;                 // sort tiles (reference points)
+    int64_t qt_total = 0;                 // sort tiles of the queries of query-cell clouds
     int64_t hb_total = 0;                 // hash buckets (reference layout)
     int64_t lt_total = 0;                 // lin-table slots
     bool any_nonself = false;
@@ -88,6 +96,7 @@ struct Work {
     int64_t *rt_start;    // [B+1] reference tile prefix
     int64_t *hb_start;    // [B+1] bucket prefix (reference-layout tables)
     int64_t *lt_start;    // [B+1] lin-table slot prefix (power-of-two per cloud)
+    int64_t *qt_start;    // [B+1] query tile prefix (query-cell clouds)
     uint32_t *misc;       // [kMiscWords]
     float4 *rwrap;        // [Nr] wrapped reference positions (+support in .w)
     uint32_t *keys0, *keys1;  // cell linear index (raw int32 bits)
@@ -118,6 +127,29 @@ struct Work {
     int2 *ranges;         // [Nr * 27] per cell: stencil cell (first global slot, count)
     float4 *recs;         // [Nr * 2] per cell: 32-byte record (slots, cloud, state, centre)
     uint32_t *big_list;   // [Nr] cells with more candidates than one register chunk
+    // Query cells of the clouds whose query set is not the reference set
+    // (searchRange 1): the queries sorted by the cell of the reference grid
+    // that holds them, so that a wavefront takes one cell's queries.  Arrays
+    // of length Nq (all queries of the batch; only the query-cell clouds'
+    // entries are used), empty when no cloud takes this path.
+    float4 *qwrap;        // [Nq] once-wrapped query positions (+support)
+    uint32_t *qkeys0, *qkeys1, *qvals0, *qvals1;
+    uint8_t *qpflags;     // [Nq] bit 1: the query's cell is outside the grid
+    uint32_t *qsort_hist; // [B * kMaxPasses * kRadix]
+    uint64_t *qsort_lb;   // [kMaxPasses * query tiles * kRadix]
+    uint64_t *qcell_lb;   // [query tiles]
+    float4 *qsorted;      // [Nq] sorted query positions (+support)
+    int32_t *qsort_index; // [Nq] sorted query slot -> local query index
+    int32_t *qskey;       // [Nq] sorted cell key
+    uint8_t *qsflags;     // [Nq] per sorted query slot
+    uint32_t *qcell_begin;  // [Nq+1] first sorted query slot of each query cell
+    int32_t *qcell_lin;   // [Nq] the cell's linear index
+    uint32_t *qcell_cloud;  // [Nq]
+    uint64_t *qmasks;     // [Nq * 7]
+    int2 *qranges;        // [Nq * 27]
+    float4 *qrecs;        // [Nq * 2]
+    uint32_t *qbig_list;  // [Nq]
+    int32_t *qstray_list; // [Nq]
 };
 
 // Bucket layout of the reference (hashTable [L,2] + cellTable [C,3]); used
@@ -180,7 +212,7 @@ inline void fill_kernel_end(hipStream_t s) { kernel_end(fill_events(), s); }
 // Word ranges a kernel zeroes on the side (k_prep: the sort and cell
 // look-back state of the search).
 struct ClearSpans {
-    static constexpr int kMax = 3;
+    static constexpr int kMax = 6;
     uint32_t *p[kMax];
     int64_t n[kMax];
 };
@@ -228,6 +260,14 @@ int mlm_sort_rows_launch(const Batch &b, const int64_t *offsets, uint64_t *scrat
 // Structure build (hMax, grid, keys, sort, cells, lin table).  Expects the
 // header block (descriptors, prefixes, zeroed misc) uploaded.
 int build_structure(const Batch &b, Work &w, hipStream_t s);
+// Clouds whose queries the query-cell path sorts and groups: a query set
+// that is not the reference set, searchRange 1, points on both sides.
+__host__ __device__ inline bool takes_query_cells(const tcr_cloud &h) {
+    return !h.self_search && h.search_range == 1 && h.nq > 0 && h.nr > 0;
+}
+// Query keys, query sort and query cells of those clouds (after the
+// reference structure).
+int build_queries(const Batch &b, Work &w, hipStream_t s);
 // Its stages: hMax + grid of every cloud (k_prep), and -- after the keys
 // (keys0/vals0/rwrap/pflags) are written -- sort, cells and lin table.
 int build_prep(const Batch &b, Work &w, hipStream_t s);
@@ -266,6 +306,7 @@ int build_bucket_tables(const Batch &b, Work &w, hipStream_t s);
 // hist_done: the digit histograms (and the pass count) were built by the
 // compact path's keys kernel; else k_sort_hist builds them.
 int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done = false);
+int radix_sort_queries(const Batch &b, Work &w, hipStream_t s);
 
 // Decoupled look-back state: a 64-bit word per (tile[, digit]) holding a
 // 32-bit count and a flag -- 0 not published, aggregate of the tile alone, or

@@ -66,6 +66,8 @@ struct SortPassArgs {
     const uint32_t *hist;
     uint64_t *lb;            // [ntiles * kRadix] of this pass
     uint32_t *misc;
+    int npass_slot;          // misc word holding this sort's pass count
+    int ticket_slot;         // first of its per-pass tile tickets
     int pass;
     // pass 0 empties the cell index when no k_sort_hist ran (the keys
     // kernel built the histograms before the irregular flag was final)
@@ -90,13 +92,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     __shared__ uint32_t tile_excl[kRadix];  // keys of this digit in earlier tiles of the cloud
     __shared__ uint32_t s_wtot[kRadix / kWave];
     __shared__ int64_t s_tile;
-    const uint32_t npass = a.misc[kMiscPasses];
+    const uint32_t npass = a.misc[a.npass_slot];
     if ((uint32_t)a.pass >= npass) return;
     const bool last = (uint32_t)a.pass == npass - 1;
     const int shift = a.pass * kRadixBits;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
 
-    if (threadIdx.x == 0) s_tile = atomicAdd(&a.misc[kMiscSortTicket + a.pass], 1u);
+    if (threadIdx.x == 0) s_tile = atomicAdd(&a.misc[a.ticket_slot + a.pass], 1u);
     for (int k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&wcnt[0][0])[k] = 0;
     __syncthreads();
     const int64_t tile = s_tile;
@@ -221,6 +223,8 @@ int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done) {
     a.p_start = w.r_start;
     a.hist = w.sort_hist;
     a.misc = w.misc;
+    a.npass_slot = kMiscPasses;
+    a.ticket_slot = kMiscSortTicket;
     a.rwrap = w.rwrap;
     a.pflags = w.pflags;
     a.sorted = w.sorted;
@@ -235,6 +239,42 @@ int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done) {
         a.vout = (p & 1) ? w.vals0 : w.vals1;
         a.lb = w.sort_lb + (int64_t)p * b.rt_total * kRadix;
         hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)b.rt_total), dim3(kSortThreads), 0, s, a);
+    }
+    TCR_LAUNCH_CHECK();
+    return TCR_OK;
+}
+
+// The same passes over the query keys of the query-cell clouds (histograms
+// counted by k_qkeys): sorted query positions, indices, keys and flags.
+int radix_sort_queries(const Batch &b, Work &w, hipStream_t s) {
+    if (b.qt_total == 0) return TCR_OK;
+    SortPassArgs a;
+    a.clear_index = 0;
+    a.lt_start = w.lt_start;
+    a.lintab = w.lintab;
+    a.desc = w.desc;
+    a.B = b.B;
+    a.ntiles = b.qt_total;
+    a.t_start = w.qt_start;
+    a.p_start = w.q_start;
+    a.hist = w.qsort_hist;
+    a.misc = w.misc;
+    a.npass_slot = kMiscQPasses;
+    a.ticket_slot = kMiscQSortTicket;
+    a.rwrap = w.qwrap;
+    a.pflags = w.qpflags;
+    a.sorted = w.qsorted;
+    a.sort_index = w.qsort_index;
+    a.skey = w.qskey;
+    a.sflags = w.qsflags;
+    for (int p = 0; p < kMaxPasses; ++p) {
+        a.pass = p;
+        a.kin = (p & 1) ? w.qkeys1 : w.qkeys0;
+        a.vin = (p & 1) ? w.qvals1 : w.qvals0;
+        a.kout = (p & 1) ? w.qkeys0 : w.qkeys1;
+        a.vout = (p & 1) ? w.qvals0 : w.qvals1;
+        a.lb = w.qsort_lb + (int64_t)p * b.qt_total * kRadix;
+        hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)b.qt_total), dim3(kSortThreads), 0, s, a);
     }
     TCR_LAUNCH_CHECK();
     return TCR_OK;
