@@ -243,25 +243,27 @@ __global__ void __launch_bounds__(kSortThreads) k_keys(CloudDev *__restrict__ de
     const int64_t cstart = p_start[ti.c];
     const int dim = cd.h.dim;
     bool out = false;
-    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
-        const int64_t g = ti.start + k;
-        const int64_t local = g - cstart;
-        float4 wp;
-        int32_t lin;
-        uint8_t f;
-        switch (dim) {
-            case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
-            case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
-            default: ref_key_one<3>(cd, local, wp, lin, f); break;
+    for (int base = 0; base < ti.len; base += kSortThreads) {  // uniform trip count: the runs span lanes
+        const int k = base + threadIdx.x;
+        const bool valid = k < ti.len;
+        int32_t lin = 0;
+        if (valid) {
+            const int64_t g = ti.start + k;
+            const int64_t local = g - cstart;
+            float4 wp;
+            uint8_t f;
+            switch (dim) {
+                case 1: ref_key_one<1>(cd, local, wp, lin, f); break;
+                case 2: ref_key_one<2>(cd, local, wp, lin, f); break;
+                default: ref_key_one<3>(cd, local, wp, lin, f); break;
+            }
+            rwrap[g] = wp;
+            keys[g] = (uint32_t)lin;
+            vals[g] = (uint32_t)local;
+            pflags[g] = f;
+            out = out || (f & 2) != 0;
         }
-        rwrap[g] = wp;
-        keys[g] = (uint32_t)lin;
-        vals[g] = (uint32_t)local;
-        pflags[g] = f;
-        out = out || (f & 2) != 0;
-#pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p)
-            atomicAdd(&h[p][((uint32_t)lin >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+        hist_add_runs(h, (uint32_t)lin, valid);
     }
     if (out) s_out = 1u;
     __syncthreads();
@@ -539,24 +541,26 @@ __global__ void __launch_bounds__(kSortThreads) k_qkeys(CloudDev *__restrict__ d
     const CloudDev &cd = desc[ti.c];
     const int64_t cstart = p_start[ti.c];
     const int dim = cd.h.dim;
-    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
-        const int64_t g = ti.start + k;
-        const int64_t local = g - cstart;
-        float4 wp;
-        int32_t lin;
-        uint8_t f;
-        switch (dim) {
-            case 1: query_key_one<1>(cd, local, wp, lin, f); break;
-            case 2: query_key_one<2>(cd, local, wp, lin, f); break;
-            default: query_key_one<3>(cd, local, wp, lin, f); break;
+    for (int base = 0; base < ti.len; base += kSortThreads) {  // uniform trip count: the runs span lanes
+        const int k = base + threadIdx.x;
+        const bool valid = k < ti.len;
+        int32_t lin = 0;
+        if (valid) {
+            const int64_t g = ti.start + k;
+            const int64_t local = g - cstart;
+            float4 wp;
+            uint8_t f;
+            switch (dim) {
+                case 1: query_key_one<1>(cd, local, wp, lin, f); break;
+                case 2: query_key_one<2>(cd, local, wp, lin, f); break;
+                default: query_key_one<3>(cd, local, wp, lin, f); break;
+            }
+            qwrap[g] = wp;
+            keys[g] = (uint32_t)lin;
+            vals[g] = (uint32_t)local;
+            pflags[g] = f;
         }
-        qwrap[g] = wp;
-        keys[g] = (uint32_t)lin;
-        vals[g] = (uint32_t)local;
-        pflags[g] = f;
-#pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p)
-            atomicAdd(&h[p][((uint32_t)lin >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+        hist_add_runs(h, (uint32_t)lin, valid);
     }
     __syncthreads();
     uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;

@@ -456,4 +456,27 @@ __device__ __forceinline__ int hist_digit(int d, int pass, bool wide) {
     return (wide && pass == kFlipPass) ? (int)((uint32_t)d ^ kFlipDigit) : d;
 }
 
+// Adds the raw digits of every pass of `key` to a tile's LDS histograms
+// h[pass][digit] with one atomic per run of equal digits among the
+// wavefront's lanes: the lanes hold consecutive keys, which share digits in
+// long runs when the points come in spatial order (a single atomic per lane
+// then serialises up to 64 updates of one word).  Every lane of the
+// wavefront calls it; the valid lanes form a prefix.
+__device__ __forceinline__ void hist_add_runs(uint32_t (*h)[kRadix], uint32_t key, bool valid) {
+    const int lane = __lane_id();
+    const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+#pragma unroll
+    for (int p = 0; p < kMaxPasses; ++p) {
+        const uint32_t d = valid ? (key >> (p * kRadixBits)) & (kRadix - 1) : 0xffffffffu;
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, kWave);
+        const bool head = valid && (lane == 0 || prev != d);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+            const uint64_t after = lane == kWave - 1 ? 0ull : (heads >> (lane + 1));
+            const uint32_t next = after ? (uint32_t)(lane + 1 + __builtin_ctzll(after)) : nv;
+            atomicAdd(&h[p][d], next - (uint32_t)lane);
+        }
+    }
+}
+
 }  // namespace tcr

@@ -42,10 +42,11 @@ k_sort_hist(const CloudDev *__restrict__ desc, int32_t B, const int64_t *__restr
             tab[g] = ~0ull;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < ti.len; k += kSortThreads) {
-        const uint32_t rk = keys[ti.start + k];  // raw key digits (hist_digit maps a wide cloud's)
-#pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p) atomicAdd(&h[p][(rk >> (p * kRadixBits)) & (kRadix - 1)], 1u);
+    for (int base = 0; base < ti.len; base += kSortThreads) {  // uniform trip count: the runs span lanes
+        const int k = base + threadIdx.x;
+        const bool valid = k < ti.len;
+        const uint32_t rk = valid ? keys[ti.start + k] : 0u;  // raw key digits (hist_digit maps a wide cloud's)
+        hist_add_runs(h, rk, valid);
     }
     __syncthreads();
     uint32_t *hc = hist + (int64_t)ti.c * kMaxPasses * kRadix;
