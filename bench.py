@@ -10,9 +10,10 @@ host round trip in between), one host sync for the pair total, and the CSR
 tuple.  The roofline figure is the dominant kernel of the search (the one
 with the larger average time of the two big passes, the wavefront-per-cell
 count kernel k_count_small and the fill kernel k_fill_small), each timed
-with HIP events that the library records on the search stream around it
-during the timed steps; both kernels' figures and the end-to-end fraction
-are reported beside it.
+with HIP events that the library records on the search stream around it,
+in a second run of as many steps right after the timed ones (the events
+are kept out of the wall-clock steps); both kernels' figures and the
+end-to-end fraction are reported beside it.
 
 Prints one JSON line (rank 0).  The CPU baseline is the C/OpenMP oracle
 timed on this host on the same workload (BASELINE.md's protocol: one
@@ -177,19 +178,25 @@ def cpu_baseline(w, reps: int = 5, single=None):
 
 
 def timed_steps(L, step, steps: int):
-    """`steps` calls of step() between two device synchronisations, with the
-    library's HIP-event phase timing on.  Returns (ms per step, mean count
-    phase ms, mean fill kernel ms, mean count kernel ms) -- NaN for a kernel
-    that did not run."""
+    """`steps` calls of step() between two device synchronisations, timed by
+    the host clock with no instrumentation inside the searches; then the
+    same number of calls again with the library's HIP-event phase timing on
+    (the events sit between kernels on the search stream and add a few
+    microseconds each, so they are kept out of the wall-clock figure).
+    Returns (ms per step, mean count phase ms, mean fill kernel ms, mean
+    count kernel ms) -- NaN for a kernel that did not run."""
     import torch
-    L.tcr_timing_collect(None, None, None, 0, None)
-    L.tcr_timing_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    L.tcr_timing_collect(None, None, None, 0, None)
+    L.tcr_timing_enable(1)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
     L.tcr_timing_enable(0)
     cms, fms, kms = ((ctypes.c_float * steps)() for _ in range(3))
     got = ctypes.c_int32(0)

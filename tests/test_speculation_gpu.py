@@ -1,0 +1,110 @@
+"""GPU parity of repeated searches on one workspace.
+
+A repeated search on the same input tensors reuses its plan (workspace,
+offsets, descriptors), and the library launches only the radix passes and
+list kernels the previous search on that workspace needed.  When a search
+needs more (more cells after its supports shrink, points outside the domain,
+crowded cells), the library repeats it with every launch before the pair
+total is returned.  Each search here is checked against the oracle on the
+values it ran on.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from torchcompactradius_amd import DomainDescription, PointCloud, radiusSearch
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _domain(mn, mx, periodic):
+    return DomainDescription(torch.tensor(mn, dtype=torch.float32), torch.tensor(mx, dtype=torch.float32),
+                             torch.tensor(periodic, dtype=torch.bool), len(mn))
+
+
+def _check(pos_t, sup_t, dom, mn, mx, periodic, fmt, L=4096):
+    out = radiusSearch(PointCloud(pos_t, sup_t), None, mode="gather", domain=dom, hashMapLength=L,
+                       algorithm="compact", format=fmt)
+    torch.cuda.synchronize()
+    o = oracle.radius_search(pos_t.cpu().numpy(), sup_t.cpu().numpy(), domain_min=mn, domain_max=mx,
+                             periodic=periodic, mode="gather", hash_map_length=L)
+    if fmt == "coo":
+        np.testing.assert_array_equal(out.row.cpu().numpy(), o.row)
+        np.testing.assert_array_equal(out.col.cpu().numpy(), o.col)
+    else:
+        np.testing.assert_array_equal(out.indices.cpu().numpy(), o.col)
+        np.testing.assert_array_equal(out.indptr.cpu().numpy(), o.indptr)
+    return int(o.col.shape[0])
+
+
+@pytest.mark.parametrize("fmt", ["csr", "coo"])
+def test_supports_shrink_between_searches(fmt):
+    """Few cells (one radix pass), then many (two, then three passes): the
+    search after each change needs passes the one before it did not."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    mn, mx, periodic = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], [True, True, False]
+    pos = torch.from_numpy(rng.uniform(-1, 1, (n, 3)).astype(np.float32)).to(DEV)
+    sup = torch.full((n,), 0.3, dtype=torch.float32, device=DEV)   # 6^3 cells
+    dom = _domain(mn, mx, periodic)
+    for h in (0.3, 0.3, 0.3, 0.05, 0.05, 0.012, 0.012, 0.3):       # 40^3 cells, then 166^3 (> 2^18)
+        print(f"case: shrink fmt={fmt} h={h}", flush=True)
+        sup.fill_(h)
+        _check(pos, sup, dom, mn, mx, periodic, fmt)
+
+
+def test_strays_and_crowded_cells_appear():
+    """Searches without listed cells or queries, then points outside the
+    domain (stray queries) and a dense cluster (cells over the register
+    chunk), then back."""
+    rng = np.random.default_rng(12)
+    n = 30000
+    mn, mx, periodic = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], [False, True, False]
+    base = rng.uniform(-0.95, 0.95, (n, 3)).astype(np.float32)
+    pos = torch.from_numpy(base).to(DEV)
+    sup = torch.full((n,), 0.08, dtype=torch.float32, device=DEV)
+    dom = _domain(mn, mx, periodic)
+    for step in range(3):
+        print(f"case: regular step {step}", flush=True)
+        _check(pos, sup, dom, mn, mx, periodic, "csr")
+    moved = base.copy()
+    moved[:50, 0] = np.float32(-1.0) - np.float32(0.01) * np.arange(1, 51, dtype=np.float32)  # outside in x
+    moved[50:3050] = (rng.normal(0.0, 0.02, (3000, 3))).astype(np.float32)                   # a dense cluster
+    pos.copy_(torch.from_numpy(moved))
+    for step in range(2):
+        print(f"case: strays and crowded cells step {step}", flush=True)
+        p = _check(pos, sup, dom, mn, mx, periodic, "csr")
+        assert p > 0
+    pos.copy_(torch.from_numpy(base))
+    print("case: back to regular", flush=True)
+    _check(pos, sup, dom, mn, mx, periodic, "csr")
+
+
+def test_plan_graph_replay_after_speculative_searches():
+    """A captured graph (every launch) replayed on a workspace whose last
+    search was speculative: the replay's results stand alone."""
+    from torchcompactradius_amd.search import CloudSpec, SearchPlan
+    rng = np.random.default_rng(13)
+    n = 8000
+    mn, mx = [-1.0, -1.0], [1.0, 1.0]
+    pos = torch.from_numpy(rng.uniform(-1, 1, (n, 2)).astype(np.float32)).to(DEV)
+    sup = torch.full((n,), 0.05, dtype=torch.float32, device=DEV)
+    spec = CloudSpec(pos, sup, None, None, mn, mx, 0b11, "gather", 4096)
+    plan = SearchPlan([spec], want_rows=True)
+    o = oracle.radius_search(pos.cpu().numpy(), sup.cpu().numpy(), domain_min=mn, domain_max=mx,
+                             periodic=[True, True], mode="gather", hash_map_length=4096)
+    for _ in range(3):
+        r = plan.run()
+        np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+    plan.capture(int(o.col.shape[0]) + 64)
+    for _ in range(2):
+        r = plan.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+        np.testing.assert_array_equal(r.row.cpu().numpy(), o.row)
+        r = plan.run()
+        np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
+    plan.release_graph()

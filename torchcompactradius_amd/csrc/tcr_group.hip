@@ -1015,7 +1015,9 @@ static int launch_modes(const Batch &b, const GroupArgs &ga, const GenericArgs &
                                count_lds, s, ga);                                                           \
             kernel_end(count_events(), s);                                                                  \
         }                                                                                                   \
-        if (last == D * 4 + M)                                                                              \
+        if (b.skip_lists)                                                                                   \
+            ;                                                                                               \
+        else if (last == D * 4 + M)                                                                         \
             hipLaunchKernelGGL((k_group_big<D, M, FILL, true>), lgrid, block, 0, s, ga, gen);               \
         else                                                                                                \
             hipLaunchKernelGGL((k_group_big<D, M, FILL, false>), bgrid, block, 0, s, ga, gen);              \

@@ -86,6 +86,13 @@ struct Batch {
     int64_t lt_total = 0;                 // lin-table slots
     bool any_nonself = false;
     std::vector<CloudDev> dev;            // host copy of the descriptors
+    // Launches of this search.  A repeated search on one workspace launches
+    // only the radix passes and list kernels the previous search on it
+    // needed; the device still counts what this one needs, and the host
+    // repeats the search with every launch when that was more (finish_search).
+    int sort_passes = kMaxPasses;         // radix passes over the reference keys
+    int qsort_passes = kMaxPasses;        // over the query keys of query-cell clouds
+    bool skip_lists = false;              // no big-cell / stray-query launches
 };
 
 // Workspace pointers.

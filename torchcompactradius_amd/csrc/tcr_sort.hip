@@ -232,7 +232,7 @@ int radix_sort(const Batch &b, Work &w, hipStream_t s, bool hist_done) {
     a.sort_index = w.sort_index;
     a.skey = w.skey;
     a.sflags = w.sflags;
-    for (int p = 0; p < kMaxPasses; ++p) {
+    for (int p = 0; p < b.sort_passes; ++p) {
         a.pass = p;
         a.kin = (p & 1) ? w.keys1 : w.keys0;
         a.vin = (p & 1) ? w.vals1 : w.vals0;
@@ -268,7 +268,7 @@ int radix_sort_queries(const Batch &b, Work &w, hipStream_t s) {
     a.sort_index = w.qsort_index;
     a.skey = w.qskey;
     a.sflags = w.qsflags;
-    for (int p = 0; p < kMaxPasses; ++p) {
+    for (int p = 0; p < b.qsort_passes; ++p) {
         a.pass = p;
         a.kin = (p & 1) ? w.qkeys1 : w.qkeys0;
         a.vin = (p & 1) ? w.qvals1 : w.qvals0;
