@@ -66,6 +66,16 @@ __device__ __forceinline__ T uniform(T v) {
 __device__ __forceinline__ float uniform_f(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
+// Read-only data through the constant address space: a wave-uniform address
+// becomes a scalar load, which waits on the scalar counter instead of the
+// vector-memory one that also covers the kernel's earlier stores.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
+}
+
 // Lane m's value, m uniform (v_readlane).
 __device__ __forceinline__ int32_t bcast(int32_t v, uint32_t m) { return __builtin_amdgcn_readlane(v, m); }
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t m) {
@@ -350,7 +360,13 @@ struct CellLoad {
 
 __device__ __forceinline__ CellLoad load_cell(const GroupArgs &a, uint32_t cid) {
     CellLoad l;
-    l.rec = a.recs[cid];
+    const cptr<uint32_t> rw = as_const((const uint32_t *)(a.recs + uniform(cid)));
+    l.rec.gb = rw[0];
+    l.rec.ge = rw[1];
+    l.rec.cloud = rw[2];
+    l.rec.state = rw[3];
+    l.rec.center = make_float4(__uint_as_float(rw[4]), __uint_as_float(rw[5]), __uint_as_float(rw[6]),
+                               __uint_as_float(rw[7]));
     l.rng = __lane_id() < kStencil ? a.ranges[(int64_t)cid * kStencil + __lane_id()] : make_int2(0, 0);
     return l;
 }
@@ -364,7 +380,7 @@ __device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad
     v.state = uniform(l.rec.state);
     v.center = make_float4(uniform_f(l.rec.center.x), uniform_f(l.rec.center.y), uniform_f(l.rec.center.z), 0.f);
     v.wrap = uniform_f(l.rec.center.w) != 0.0f;
-    v.qs = uniform(a.q_start[v.c]);
+    v.qs = as_const(a.q_start)[v.c];
     const int32_t cnt = lane < noff ? l.rng.y : 0;
     int32_t incl = cnt;
 #pragma unroll
@@ -430,14 +446,17 @@ struct Geo {
     float ext[3], center[3];
     float box, geo, cell2;
     uint32_t pmask;
-    __device__ __forceinline__ Geo(const CloudDev &cd, float4 cen) {
-        pmask = cd.h.periodic_mask;
-        const float hcell = cd.hcell;
+    __device__ __forceinline__ Geo(const CloudDev &cd, float4 cen) { init(&cd, cen); }
+    __device__ __forceinline__ Geo(cptr<CloudDev> cd, float4 cen) { init(cd, cen); }
+    template <class P>
+    __device__ __forceinline__ void init(P cd, float4 cen) {
+        pmask = cd->h.periodic_mask;
+        const float hcell = cd->hcell;
         const float cv[3] = {cen.x, cen.y, cen.z};
         float coord_max = 0.0f, ext_max = 0.0f;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const float mn = cd.h.domain_min[d], mx = cd.h.domain_max[d];
+            const float mn = cd->h.domain_min[d], mx = cd->h.domain_max[d];
             ext[d] = mx - mn;
             center[d] = cv[d];
             coord_max = fmaxf(coord_max, fmaxf(fabsf(mn), fabsf(mx)));
@@ -598,7 +617,7 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
     int32_t mloc = a.msort_index[ms0];
     float4 mq = a.msorted[ms0];
 
-    const Geo<DIM> g(cd, v.center);
+    const Geo<DIM> g(as_const(a.desc) + v.c, v.center);
     const int nrounds = (v.total + kWave - 1) / kWave;  // 1..kSmallRounds
     expand_slots(v, slot_off, kOff);
 
@@ -704,13 +723,16 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
     // first member batch: independent of the candidates, issue it first
     const uint32_t nm0 = min((uint32_t)kWave, v.ge - v.gb);
     const uint32_t ms0 = (uint32_t)lane < nm0 ? v.gb + lane : v.gb;
-    bool live = (uint32_t)lane < nm0 && !a.msflags[ms0];
+    // masks and row offsets are read for every member (a stray member's are
+    // never used), so that their loads do not wait for the stray flags
+    const bool mv0 = (uint32_t)lane < nm0;
+    bool live = mv0 && !a.msflags[ms0];
     uint64_t mk[kSmallRounds];
 #pragma unroll
     for (int r = 0; r < kSmallRounds; ++r)
-        mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms0] : 0ull;
+        mk[r] = (r < nrounds && mv0) ? a.masks[r * a.mask_stride + ms0] : 0ull;
     int32_t mloc = a.msort_index[ms0];
-    int64_t mbase = live ? a.offsets[v.qs + mloc] : 0;
+    int64_t mbase = mv0 ? a.offsets[v.qs + mloc] : 0;
 
     expand_slots(v, slot_off, noff);
     int32_t corig[kSmallRounds];
@@ -727,9 +749,9 @@ __device__ void fill_small(const GroupArgs &a, const CellView &v, int noff, int3
             live = mvalid && !a.msflags[ms];
 #pragma unroll
             for (int r = 0; r < kSmallRounds; ++r)
-                mk[r] = (r < nrounds && live) ? a.masks[r * a.mask_stride + ms] : 0ull;
+                mk[r] = (r < nrounds && mvalid) ? a.masks[r * a.mask_stride + ms] : 0ull;
             mloc = a.msort_index[ms];
-            mbase = live ? a.offsets[v.qs + mloc] : 0;
+            mbase = mvalid ? a.offsets[v.qs + mloc] : 0;
         }
         const uint64_t todo = __ballot(live);
         if (!todo) continue;
@@ -851,7 +873,7 @@ __device__ __forceinline__ CellRange my_cells(uint32_t ncells, uint32_t run) {
     const uint32_t wpb = blockDim.x / kWave;
     const uint32_t xcd = blockIdx.x % nxcd;
     const uint32_t per_xcd_waves = (gridDim.x / nxcd) * wpb;
-    const uint32_t wix = (blockIdx.x / nxcd) * wpb + threadIdx.x / kWave;
+    const uint32_t wix = (blockIdx.x / nxcd) * wpb + uniform(threadIdx.x / kWave);
     CellRange r;
     r.run = run > 0 ? run : 1u;
     const uint32_t c0 = (uint32_t)(((uint64_t)ncells * xcd) / nxcd);
@@ -863,6 +885,9 @@ __device__ __forceinline__ CellRange my_cells(uint32_t ncells, uint32_t run) {
 
 __device__ __forceinline__ bool cloud_is(const CloudDev &cd, int dim, int mode) {
     return cd.h.dim == dim && (mode < 0 || cd.h.mode == mode);
+}
+__device__ __forceinline__ bool cloud_is(cptr<CloudDev> cd, int dim, int mode) {
+    return cd->h.dim == dim && (mode < 0 || cd->h.mode == mode);
 }
 
 // 5 waves per SIMD: the register budget (<= 102 VGPRs) spills only per-cell
@@ -880,7 +905,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
     // that hides the same latency
     for (uint32_t cid = cr.first, k = 1; cid < cr.end; cid = cr.next(cid, k)) {
         const CellView v = make_view(a, load_cell(a, cid), kOff);
-        if (v.state != kCellSkip && cloud_is(a.desc[v.c], DIM, MODE)) {
+        if (v.state != kCellSkip && cloud_is(as_const(a.desc) + v.c, DIM, MODE)) {
             if (v.state == kCellStray) {
                 append_strays(a, v);
             } else if (v.total > kSmallChunk) {
@@ -908,7 +933,7 @@ __global__ void __launch_bounds__(kGroupThreads) __attribute__((amdgpu_waves_per
         const CellLoad cur = load_cell(a, cid);
         const uint32_t state = uniform(cur.rec.state);
         if (state == kCellFast) {
-            const int dim = a.desc[uniform(cur.rec.cloud)].h.dim;
+            const int dim = as_const(a.desc)[uniform(cur.rec.cloud)].h.dim;
             const int noff = dim == 1 ? 3 : dim == 2 ? 9 : 27;
             const CellView v = make_view(a, cur, noff);
             if (v.total <= kSmallChunk)  // bigger cells: k_group_big
