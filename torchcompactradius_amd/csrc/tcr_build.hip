@@ -54,6 +54,22 @@ __device__ void grid_of(CloudDev &cd, uint32_t hmax_ord) {
     float hc = h[0];
     for (int d = 1; d < dim; ++d) hc = (h[d] > hc || isnan(h[d])) ? h[d] : hc;
     cd.hcell = hc;
+    {
+        // the filter geometry: candidates of a cell's stencil lie within
+        // 1.5 cells of its centre, queries of the cell within 0.5 cells
+        float coord_max = 0.0f, ext_max = 0.0f;
+        for (int d = 0; d < 3; ++d) cd.f_ext[d] = 0.0f;
+        for (int d = 0; d < dim; ++d) {
+            const float mn = cd.h.domain_min[d], mx = cd.h.domain_max[d];
+            cd.f_ext[d] = mx - mn;
+            coord_max = fmaxf(coord_max, fmaxf(fabsf(mn), fabsf(mx)));
+            if ((cd.h.periodic_mask >> d) & 1u) ext_max = fmaxf(ext_max, cd.f_ext[d]);
+        }
+        cd.f_box = 1.51f * hc;
+        cd.f_geo = 3.0f * (coord_max + 2.0f * cd.f_box) + ext_max + 3.0f * hc;
+        const float cell2 = hc * hc;
+        cd.f_c80 = 80.0f * cell2;
+    }
     uint32_t err = 0;
     // wavefront-per-cell eligibility: stencil radius 1, a finite positive
     // cell size, and >= 5 cells in every periodic dimension (the cells are

@@ -35,6 +35,12 @@ struct CloudDev {
     int64_t lt_off, lt_cap;   // lin -> cell table slots of this cloud (cap: power of two)
     int64_t cell_off, ncells; // cells of this cloud in the batch cell arrays
     int64_t qcell_off, nqcells; // query groups (non-self) in the batch
+    // filter geometry of the wavefront-per-cell kernels (grid_of): the
+    // periodic extents, the stencil box half-width, the coordinate bound of
+    // the error margin and its centring term 80 hCell^2
+    float f_ext[3];
+    float f_box, f_geo, f_c80;
+    float f_pad[2];
 };
 
 // ------------------------------------------------------------------ utils --

@@ -109,7 +109,7 @@ __device__ __forceinline__ int32_t lane_rank(uint64_t bits, int32_t base) {
 struct Thr {
     float lo, hi;
 };
-__device__ __forceinline__ Thr thresholds(float h, float geo, float cell2, int dim) {
+__device__ __forceinline__ Thr thresholds(float h, float geo, float c80, int dim) {
     Thr t;
     if (!(h > 0.0f)) {          // h <= 0 or NaN: no distance is below it
         t.lo = -INFINITY;
@@ -126,7 +126,7 @@ __device__ __forceinline__ Thr thresholds(float h, float geo, float cell2, int d
             // twice the bound: 80u cell^2 (centring + expansion), 4u h^2
             // (both accumulations, sqrtf, h*h), 2 dim h u geo (fp32
             // differences and image shifts of the reference and the estimate)
-            const float m = 2.0f * kU * (80.0f * cell2 + 4.0f * h2 + 2.0f * (float)dim * h * geo);
+            const float m = 2.0f * kU * (c80 + 4.0f * h2 + 2.0f * (float)dim * h * geo);
             t.lo = h2 - m;
             t.hi = h2 + m;
         }
@@ -349,7 +349,8 @@ struct CellView {
     bool wrap;           // some stencil candidates are periodic images
     float4 center;
     int64_t qs;          // first global query index of the cloud
-    int32_t rbeg, pre;   // per lane (stencil cell `lane`)
+    int32_t rbeg, pre;   // per lane (stencil cell `lane`): first global slot, candidate prefix
+    int32_t cnt;         // per lane: points of stencil cell `lane` (0 past the stencil)
     int32_t total;       // candidates
 };
 
@@ -390,6 +391,7 @@ __device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad
     }
     v.rbeg = l.rng.x;
     v.pre = incl - cnt;
+    v.cnt = cnt;
     v.total = uniform(__shfl(incl, noff - 1, kWave));
     return v;
 }
@@ -412,9 +414,7 @@ __device__ __forceinline__ int32_t candidate_slot(const CellView &v, int32_t k, 
 // candidate k lives at global slot k + slot_off[k].  Lane o writes the
 // entries of stencil cell o.
 __device__ __forceinline__ void expand_slots(const CellView &v, int32_t *slot_off, int noff) {
-    const int lane = __lane_id();
-    const int32_t next = __shfl_down(v.pre, 1, kWave);
-    const int32_t cnt = lane < noff ? (lane == noff - 1 ? v.total - v.pre : next - v.pre) : 0;
+    const int32_t cnt = v.cnt;
     const int32_t d = v.rbeg - v.pre;
     for (int32_t t = 0; __any(t < cnt); ++t)
         if (t < cnt) slot_off[v.pre + t] = d;
@@ -444,29 +444,23 @@ __device__ __forceinline__ void append_flagged(const GroupArgs &a, const CellVie
 template <int DIM>
 struct Geo {
     float ext[3], center[3];
-    float box, geo, cell2;
+    float box, geo, c80;  // c80 = 80 hCell^2, the centring term of the margin
     uint32_t pmask;
     __device__ __forceinline__ Geo(const CloudDev &cd, float4 cen) { init(&cd, cen); }
     __device__ __forceinline__ Geo(cptr<CloudDev> cd, float4 cen) { init(cd, cen); }
+    // the per-cloud parts come from the descriptor (computed by grid_of)
     template <class P>
     __device__ __forceinline__ void init(P cd, float4 cen) {
         pmask = cd->h.periodic_mask;
-        const float hcell = cd->hcell;
         const float cv[3] = {cen.x, cen.y, cen.z};
-        float coord_max = 0.0f, ext_max = 0.0f;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const float mn = cd->h.domain_min[d], mx = cd->h.domain_max[d];
-            ext[d] = mx - mn;
+            ext[d] = cd->f_ext[d];
             center[d] = cv[d];
-            coord_max = fmaxf(coord_max, fmaxf(fabsf(mn), fabsf(mx)));
-            if ((pmask >> d) & 1u) ext_max = fmaxf(ext_max, ext[d]);
         }
-        // candidates of the stencil lie within 1.5 cells of the centre,
-        // queries of the cell within 0.5 cells
-        box = 1.51f * hcell;
-        geo = 3.0f * (coord_max + 2.0f * box) + ext_max + 3.0f * hcell;
-        cell2 = hcell * hcell;
+        box = cd->f_box;
+        geo = cd->f_geo;
+        c80 = cd->f_c80;
     }
     // Candidate at its image nearest to the cell, centred, with |c|^2 in p:
     // +inf for a lane without a candidate, NaN for a candidate outside the
@@ -547,8 +541,8 @@ __device__ __forceinline__ uint64_t round_hits(const GroupArgs &a, const CloudDe
         th = tq;
     } else {
         if (MODE == TCR_MODE_SCATTER) th = tjr;
-        else if (MODE == TCR_MODE_SYMMETRIC) th = thresholds((hi + ch) / 2.f, g.geo, g.cell2, DIM);
-        else th = thresholds(max_ref(hi, ch), g.geo, g.cell2, DIM);
+        else if (MODE == TCR_MODE_SYMMETRIC) th = thresholds((hi + ch) / 2.f, g.geo, g.c80, DIM);
+        else th = thresholds(max_ref(hi, ch), g.geo, g.c80, DIM);
         t += pi;
     }
     uint64_t hitm = __ballot(t < th.lo);
@@ -637,7 +631,7 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
         if (v.wrap) g.place(cv, valid, cx[r], cp[r]);
         else g.place_direct(cv, valid, cx[r], cp[r]);
         chh[r] = cv.w;
-        if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
+        if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.c80, DIM);
     }
 
     for (uint32_t mb = v.gb; mb < v.ge; mb += kWave) {
@@ -666,7 +660,7 @@ __device__ void count_small(const GroupArgs &a, const CellView &v, uint32_t cid,
         }
         Thr mt{0.f, 0.f};
         if (MODE == TCR_MODE_GATHER) {
-            mt = thresholds(mq.w, g.geo, g.cell2, DIM);
+            mt = thresholds(mq.w, g.geo, g.c80, DIM);
             mt.lo -= mpi;
             mt.hi -= mpi;
         }
@@ -802,7 +796,7 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
             const float4 cv = a.sorted[valid ? j : 0];
             g.place(cv, valid, cx[r], cp[r]);
             chh[r] = cv.w;
-            if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.cell2, DIM);
+            if (MODE == TCR_MODE_SCATTER) tj[r] = thresholds(cv.w, g.geo, g.c80, DIM);
         }
         for (uint32_t s = v.gb; s < v.ge; ++s) {
             if (a.msflags[s]) continue;  // listed for the per-query kernel by k_count_small
@@ -820,7 +814,7 @@ __device__ void group_big(const GroupArgs &a, uint32_t cid) {
             const float hi = mq.w;
             Thr tq{0.f, 0.f};
             if (MODE == TCR_MODE_GATHER) {
-                tq = thresholds(hi, g.geo, g.cell2, DIM);
+                tq = thresholds(hi, g.geo, g.c80, DIM);
                 tq.lo -= pi;
                 tq.hi -= pi;
             }
