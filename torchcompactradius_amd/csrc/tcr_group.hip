@@ -372,6 +372,18 @@ __device__ __forceinline__ CellLoad load_cell(const GroupArgs &a, uint32_t cid) 
     return l;
 }
 
+// Inclusive prefix sum over lanes 0..31 (the stencil cells) with DPP row
+// shifts and the row-15 broadcast: no lane-address registers, which the
+// shuffle form keeps live across the whole cell loop.
+__device__ __forceinline__ int32_t scan_lanes32(int32_t x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1 and 3
+    return x;
+}
+
 __device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad &l, int noff) {
     const int lane = __lane_id();
     CellView v;
@@ -383,16 +395,11 @@ __device__ __forceinline__ CellView make_view(const GroupArgs &a, const CellLoad
     v.wrap = uniform_f(l.rec.center.w) != 0.0f;
     v.qs = as_const(a.q_start)[v.c];
     const int32_t cnt = lane < noff ? l.rng.y : 0;
-    int32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const int32_t n = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += n;
-    }
+    const int32_t incl = scan_lanes32(cnt);
     v.rbeg = l.rng.x;
     v.pre = incl - cnt;
     v.cnt = cnt;
-    v.total = uniform(__shfl(incl, noff - 1, kWave));
+    v.total = bcast(incl, (uint32_t)(noff - 1));
     return v;
 }
 
