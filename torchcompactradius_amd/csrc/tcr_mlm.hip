@@ -271,6 +271,7 @@ __device__ __forceinline__ float level_radius(int mode, float hi, float hl) {
 // offset; or emit -- count them and write their keys into the query's slot
 // (rows longer than a slot raise kMlmSlotOverflow).
 enum : int { kMlmCount = 0, kMlmFill = 1, kMlmEmit = 2 };
+constexpr int kMlmBatch = 8;  // candidates loaded together
 
 template <int DIM, int PASS>
 __device__ void mlm_query(const MlmArgs &a, int64_t g) {
@@ -305,16 +306,33 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
     int64_t k = 0;
     const int64_t base = FILL ? a.offsets[g] : 0;
 
-    auto test = [&](uint32_t j, int o) {
-        const float4 v = a.sorted[j];
+    auto test_v = [&](const float4 &v, int32_t sidx, int o) {
         const float xj[3] = {v.x, v.y, v.z};
         const float dist = exact_distance<DIM>(xi, xj, mn, mx, pmask);
         if (!support_pred(dist, hi, v.w, mode)) return;
-        const uint64_t key = ((uint64_t)(uint32_t)o << 32) | (uint32_t)a.sort_index[j];
+        const uint64_t key = ((uint64_t)(uint32_t)o << 32) | (uint32_t)sidx;
         if (FILL) a.scratch[base + k] = key;
         if (PASS == kMlmEmit && k < a.slot_cap)
-            a.slots[g * a.slot_cap + k] = ((uint32_t)o << kMlmKey32Shift) | (uint32_t)a.sort_index[j];
+            a.slots[g * a.slot_cap + k] = ((uint32_t)o << kMlmKey32Shift) | (uint32_t)sidx;
         ++k;
+    };
+    // candidates [jb, je) in batches whose loads are all in flight before
+    // the first is tested (a hit's store would otherwise make every next
+    // load wait); the tail re-reads the last candidate and skips it
+    auto test_run = [&](uint32_t jb, uint32_t je, int o) {
+        for (uint32_t j = jb; j < je; j += kMlmBatch) {
+            float4 v[kMlmBatch];
+            int32_t si[kMlmBatch];
+#pragma unroll
+            for (int u = 0; u < kMlmBatch; ++u) {
+                const uint32_t jj = min(j + (uint32_t)u, je - 1);
+                v[u] = a.sorted[jj];
+                si[u] = a.sort_index[jj];
+            }
+#pragma unroll
+            for (int u = 0; u < kMlmBatch; ++u)
+                if (j + (uint32_t)u < je) test_v(v[u], si[u], o);
+        }
     };
 
     for (int l = 0; l < a.L; ++l) {
@@ -365,7 +383,7 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) nsub *= (int64_t)(shi[d] - slo[d] + 1);
             if ((int64_t)(ce - cb) <= 2 * nsub) {
-                for (uint32_t j = cb; j < ce; ++j) test(j, o);
+                test_run(cb, ce, o);
             } else {
                 for (int32_t s2 = slo[DIM > 2 ? 2 : 0]; s2 <= (DIM > 2 ? shi[2] : slo[0]); ++s2)
                     for (int32_t s1 = slo[DIM > 1 ? 1 : 0]; s1 <= (DIM > 1 ? shi[1] : slo[0]); ++s1)
@@ -374,7 +392,7 @@ __device__ void mlm_query(const MlmArgs &a, int64_t g) {
                             const int32_t cid = lintab_find(tab, cap, (int32_t)fine_key<DIM>(l, clin, sub_index<DIM>(sub, l)));
                             if (cid < 0) continue;
                             const uint32_t jb = a.cell_begin[cid], je = a.cell_begin[cid + 1];
-                            for (uint32_t j = jb; j < je; ++j) test(j, o);
+                            if (jb < je) test_run(jb, je, o);
                         }
             }
         }
