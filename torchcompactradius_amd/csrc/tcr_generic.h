@@ -8,6 +8,7 @@
 namespace tcr {
 
 constexpr int kQBlock = 256;
+constexpr int kTraverseBatch = 4;  // candidates a per-query traversal loads together
 
 // Candidate storage of the batched search: sorted float4 (x, y, z, h).
 struct Float4Cands {
@@ -94,13 +95,22 @@ __device__ __forceinline__ int64_t traverse(const Geom &g, const Lookup &lk, con
         }
         int64_t cb, ce;
         if (!lk.find(linear_index<DIM>(c, g.nc), hash_index<DIM>(c, g.L), cb, ce)) continue;
-        for (int64_t j = cb; j < ce; ++j) {
-            float xj[3], hj;
-            cands.get(j, xj, hj);
-            const float dist = exact_distance<DIM>(xi, xj, g.mn, g.mx, g.pmask);
-            if (support_pred(dist, hi, hj, g.mode)) {
-                emit(k, j);
-                ++k;
+        // candidates in batches whose loads are all in flight before the
+        // first is tested (an emitted hit's store would otherwise make every
+        // next load wait); the tail re-reads the cell's last candidate and
+        // skips it
+        for (int64_t j = cb; j < ce; j += kTraverseBatch) {
+            float xj[kTraverseBatch][3], hj[kTraverseBatch];
+#pragma unroll
+            for (int u = 0; u < kTraverseBatch; ++u) cands.get(min(j + u, ce - 1), xj[u], hj[u]);
+#pragma unroll
+            for (int u = 0; u < kTraverseBatch; ++u) {
+                if (j + u >= ce) break;
+                const float dist = exact_distance<DIM>(xi, xj[u], g.mn, g.mx, g.pmask);
+                if (support_pred(dist, hi, hj[u], g.mode)) {
+                    emit(k, j + u);
+                    ++k;
+                }
             }
         }
     }
