@@ -1,11 +1,11 @@
 """GPU parity of repeated searches on one workspace.
 
 A repeated search on the same input tensors reuses its plan (workspace,
-offsets, descriptors), and the library launches only the radix passes and
-list kernels the previous search on that workspace needed.  When a search
-needs more (more cells after its supports shrink, points outside the domain,
-crowded cells), the library repeats it with every launch before the pair
-total is returned.  Each search here is checked against the oracle on the
+offsets, descriptors), and the library launches only the radix passes, list
+kernels and per-query kernel the previous search on that workspace needed.
+When a search needs more (more cells after its supports shrink, points
+outside the domain, crowded cells, a grid the cell kernels do not take), the
+library repeats it with every launch before the pair total is returned.  Each search here is checked against the oracle on the
 values it ran on.
 """
 import numpy as np
@@ -108,3 +108,30 @@ def test_plan_graph_replay_after_speculative_searches():
         r = plan.run()
         np.testing.assert_array_equal(r.col.cpu().numpy(), o.col)
     plan.release_graph()
+
+
+def test_query_cells_lose_the_cell_path():
+    """A search whose query set is not the reference set: several searches
+    on the query-cell path, then supports large enough that a periodic
+    dimension has fewer than 5 cells (the per-query kernel takes the
+    queries), then back."""
+    rng = np.random.default_rng(14)
+    nq, nr = 6000, 7000
+    mn, mx, periodic = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], [True, False, True]
+    q = torch.from_numpy(rng.uniform(-1, 1, (nq, 3)).astype(np.float32)).to(DEV)
+    r = torch.from_numpy(rng.uniform(-1, 1, (nr, 3)).astype(np.float32)).to(DEV)
+    hq = torch.full((nq,), 0.1, dtype=torch.float32, device=DEV)
+    hr = torch.full((nr,), 0.1, dtype=torch.float32, device=DEV)
+    dom = _domain(mn, mx, periodic)
+    for h in (0.1, 0.1, 0.1, 0.45, 0.45, 0.1):      # 20 cells per dimension, then 4
+        print(f"case: query cells h={h}", flush=True)
+        hq.fill_(h)
+        hr.fill_(h)
+        out = radiusSearch(PointCloud(q, hq), PointCloud(r, hr), mode="gather", domain=dom, hashMapLength=4096,
+                           algorithm="compact", format="csr")
+        torch.cuda.synchronize()
+        o = oracle.radius_search(q.cpu().numpy(), hq.cpu().numpy(), r.cpu().numpy(), hr.cpu().numpy(),
+                                 domain_min=mn, domain_max=mx, periodic=periodic, mode="gather",
+                                 hash_map_length=4096)
+        np.testing.assert_array_equal(out.indices.cpu().numpy(), o.col)
+        np.testing.assert_array_equal(out.indptr.cpu().numpy(), o.indptr)

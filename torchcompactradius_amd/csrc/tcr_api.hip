@@ -242,6 +242,7 @@ struct Readback {
     uint32_t passes;   // radix passes the reference keys needed
     uint32_t qpasses;  // radix passes the query keys needed
     uint32_t lists;    // nonzero: big cells or stray queries were listed
+    uint32_t generic;  // nonzero: some cloud's queries needed the per-query kernel
 };
 static Readback *readback_slot() {
     static thread_local Readback *slot = nullptr;
@@ -258,6 +259,7 @@ __global__ void k_readback(const int64_t *__restrict__ total, const uint32_t *__
         rb->passes = misc[kMiscPasses];
         rb->qpasses = misc[kMiscQPasses];
         rb->lists = misc[kMiscBig] | misc[kMiscStrays] | misc[kMiscQBig] | misc[kMiscQStrays];
+        rb->generic = misc[kMiscGeneric];
         __threadfence_system();
     }
 }
@@ -274,9 +276,10 @@ struct SpecRecord {
     int64_t nq = 0, nr = 0;
     uint32_t passes = kMaxPasses, qpasses = kMaxPasses;
     int clean = 0;        // consecutive searches that listed no big cell or stray query
+    bool generic = true;  // the last search needed the per-query kernel
     bool pending = false; // the last launch skipped something; its arguments follow
     int sort_passes = kMaxPasses, qsort_passes = kMaxPasses;
-    bool skip_lists = false;
+    bool skip_lists = false, skip_generic = false;
     std::vector<tcr_cloud> clouds;
     const float *qpos = nullptr, *qsup = nullptr, *rpos = nullptr, *rsup = nullptr;
     size_t ws_bytes = 0;
@@ -299,6 +302,7 @@ static void speculate(const void *ws, Batch &b) {
     b.sort_passes = (int)std::min<uint32_t>(kMaxPasses, std::max<uint32_t>(1, r.passes));
     b.qsort_passes = (int)std::min<uint32_t>(kMaxPasses, std::max<uint32_t>(1, r.qpasses));
     b.skip_lists = r.clean >= 2;
+    b.skip_generic = !r.generic;
 }
 
 // Optional per-thread phase timing: HIP events recorded on the search
@@ -353,7 +357,8 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     }
     if (spec) speculate(ws, p.b);
     {
-        const bool skipped = p.b.sort_passes < kMaxPasses || p.b.qsort_passes < kMaxPasses || p.b.skip_lists;
+        const bool skipped = p.b.sort_passes < kMaxPasses || p.b.qsort_passes < kMaxPasses || p.b.skip_lists ||
+                             p.b.skip_generic;
         std::lock_guard<std::mutex> lock(g_spec_mu);
         auto it = g_spec.find(ws);
         if (skipped) {
@@ -366,6 +371,7 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
             r.sort_passes = p.b.sort_passes;
             r.qsort_passes = p.b.qsort_passes;
             r.skip_lists = p.b.skip_lists;
+            r.skip_generic = p.b.skip_generic;
             r.clouds.assign(clouds, clouds + n);
             r.qpos = qpos;
             r.qsup = qsup;
@@ -462,7 +468,7 @@ static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *tot
         }
         SpecRecord &r = it->second;
         if (r.pending && (rb->passes > (uint32_t)r.sort_passes || rb->qpasses > (uint32_t)r.qsort_passes ||
-                          (r.skip_lists && rb->lists))) {
+                          (r.skip_lists && rb->lists) || (r.skip_generic && rb->generic))) {
             again = true;
             redo = r;
         }
@@ -473,6 +479,7 @@ static int finish_search(const Prepared &p, const int64_t *offsets, int64_t *tot
         r.passes = rb->passes;
         r.qpasses = rb->qpasses;
         r.clean = rb->lists ? 0 : std::min(r.clean + 1, 1 << 20);
+        r.generic = rb->generic != 0;
     }
     if (again) {
         Prepared q;

@@ -203,6 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_grid(CloudDev *__restrict__ desc, in
     for (int c = blockIdx.x * kBlock + threadIdx.x; c < B; c += gridDim.x * kBlock) {
         grid_of(desc[c], desc[c].hmax_ord);
         if (desc[c].err) atomicOr(&misc[kMiscErr], 1u);
+        if (desc[c].h.nq > 0 && !owned_by_cells(desc[c])) atomicOr(&misc[kMiscGeneric], 1u);
     }
 }
 

@@ -1173,7 +1173,8 @@ static int query_pass(const Batch &b, Work &w, const int64_t *offsets, PairOut o
         if (rc) return rc;
     }
     // every other cloud, and query-cell clouds the cell kernels do not take
-    if (any_other || any_qcells) return launch_generic(b, w, offsets, out, capacity, fill, s);
+    // (none, when the launch history says so: skip_generic)
+    if (any_other || (any_qcells && !b.skip_generic)) return launch_generic(b, w, offsets, out, capacity, fill, s);
     return TCR_OK;
 }
 

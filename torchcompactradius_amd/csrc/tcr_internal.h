@@ -39,6 +39,7 @@ enum : int {
     kMiscQStrays = 19,     // queries the query-cell count pass leaves to the per-query path
     kMiscQBig = 20,        // query cells with more candidates than one register chunk
     kMiscQSortTicket = 24, // tile tickets of the query radix passes (kMaxPasses)
+    kMiscGeneric = 28,     // nonzero: some cloud's queries need the per-query kernel (k_grid)
     kMiscWords = 32,
 };
 
@@ -93,6 +94,7 @@ struct Batch {
     int sort_passes = kMaxPasses;         // radix passes over the reference keys
     int qsort_passes = kMaxPasses;        // over the query keys of query-cell clouds
     bool skip_lists = false;              // no big-cell / stray-query launches
+    bool skip_generic = false;            // no per-query kernel for the query-cell clouds
 };
 
 // Workspace pointers.
@@ -271,6 +273,14 @@ int build_structure(const Batch &b, Work &w, hipStream_t s);
 // that is not the reference set, searchRange 1, points on both sides.
 __host__ __device__ inline bool takes_query_cells(const tcr_cloud &h) {
     return !h.self_search && h.search_range == 1 && h.nq > 0 && h.nr > 0;
+}
+// Clouds whose queries the wavefront-per-cell kernels own (their strays
+// come back through the stray lists): self-searches of searchRange 1, and
+// the query-cell clouds whose grid the cell kernels take (k_grid's fast
+// flag).
+__device__ __forceinline__ bool owned_by_cells(const CloudDev &cd) {
+    if (cd.h.self_search) return cd.h.search_range == 1;
+    return takes_query_cells(cd.h) && cd.fast;
 }
 // Query keys, query sort and query cells of those clouds (after the
 // reference structure).

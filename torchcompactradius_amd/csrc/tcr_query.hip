@@ -14,15 +14,6 @@
 
 namespace tcr {
 
-// Clouds whose queries the wavefront-per-cell kernels own (their strays
-// come back through the stray lists): self-searches of searchRange 1, and
-// the query-cell clouds whose grid the cell kernels take (k_grid's fast
-// flag; see takes_query_cells).
-__device__ __forceinline__ bool owned_by_cells(const CloudDev &cd) {
-    if (cd.h.self_search) return cd.h.search_range == 1;
-    return takes_query_cells(cd.h) && cd.fast;
-}
-
 // Every query of the clouds the wavefront-per-cell kernels do not own.
 template <bool FILL>
 __global__ void __launch_bounds__(kQBlock) k_query_generic(GenericArgs a) {
