@@ -410,8 +410,8 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     if (rc) return rc;
     // the scan writes the CSR arrays too (before the fill, which the
     // device skips when the total exceeds the capacity)
-    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s, csr.indptr,
-                                          csr.row_entries);
+    rc = offsets_scan(p.w.counts, offsets, p.b.nq_total, p.w.scan_part, p.w.misc + kMiscScanTicket, s, csr.indptr,
+                      csr.row_entries);
     if (rc) return rc;
     if (capacity >= 0) {
         if (!out.col) {

@@ -454,6 +454,8 @@ int build_prep(const Batch &b, Work &w, hipStream_t s) {
         clear.p[5] = (uint32_t *)w.qcell_lb;
         clear.n[5] = 2 * b.qt_total;
     }
+    clear.p[6] = (uint32_t *)w.scan_part;  // offsets-scan look-back status
+    clear.n[6] = 2 * offsets_scan_tiles(b.nq_total);
     hipLaunchKernelGGL(k_prep, dim3((unsigned)std::max(1, nbq + nbr)), dim3(kBlock), 0, s, w.desc, b.B,
                        (const int64_t *)w.q_start, b.nq_total, nbq, (const int64_t *)w.r_start, b.nr_total, nbr,
                        clear);

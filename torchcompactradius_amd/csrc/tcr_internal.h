@@ -40,6 +40,7 @@ enum : int {
     kMiscQBig = 20,        // query cells with more candidates than one register chunk
     kMiscQSortTicket = 24, // tile tickets of the query radix passes (kMaxPasses)
     kMiscGeneric = 28,     // nonzero: some cloud's queries need the per-query kernel (k_grid)
+    kMiscScanTicket = 29,  // tile ticket of the offsets scan
     kMiscWords = 32,
 };
 
@@ -218,10 +219,10 @@ inline void kernel_end(KernelEvents &k, hipStream_t s) {
 inline void fill_kernel_begin(hipStream_t s) { kernel_begin(fill_events(), s); }
 inline void fill_kernel_end(hipStream_t s) { kernel_end(fill_events(), s); }
 
-// Word ranges a kernel zeroes on the side (k_prep: the sort and cell
-// look-back state of the search).
+// Word ranges a kernel zeroes on the side (k_prep: the sort, cell and
+// offsets-scan look-back state of the search).
 struct ClearSpans {
-    static constexpr int kMax = 6;
+    static constexpr int kMax = 7;
     uint32_t *p[kMax];
     int64_t n[kMax];
 };
@@ -234,6 +235,13 @@ template <typename Tin, typename Tout>
 int exclusive_scan(const Tin *in, Tout *out, int64_t n, bool write_total, uint64_t *partials,
                    hipStream_t s, int32_t *indptr = nullptr, int64_t *row_entries = nullptr);
 size_t scan_partials_needed(int64_t n);
+// Single-launch form for the counts -> int64 offsets scan of a search: tiles
+// take tickets from *ticket and chain their prefixes by decoupled look-back
+// over status[0 .. tiles); both must be zero when the launch starts (the
+// status words fit in scan_partials_needed(n) words).
+int offsets_scan(const int32_t *counts, int64_t *offsets, int64_t n, uint64_t *status, uint32_t *ticket,
+                 hipStream_t s, int32_t *indptr = nullptr, int64_t *row_entries = nullptr);
+int64_t offsets_scan_tiles(int64_t n);
 int csr_arrays_launch(const int64_t *offsets, int64_t n, int32_t *indptr, int64_t *row_entries, hipStream_t s);
 
 // Launch-shape parameters (tcr_set_tuning); 0 = automatic.
