@@ -135,3 +135,74 @@ def test_query_cells_lose_the_cell_path():
                                  hash_map_length=4096)
         np.testing.assert_array_equal(out.indices.cpu().numpy(), o.col)
         np.testing.assert_array_equal(out.indptr.cpu().numpy(), o.indptr)
+
+
+def test_fill_repeated_after_one_count():
+    """The fill pass launched several times after one count pass (each
+    launch of a cell kernel takes its cells from a counter the previous
+    launch on that counter has put back)."""
+    import ctypes
+    from torchcompactradius_amd import search as S
+    from torchcompactradius_amd._lib import check
+    rng = np.random.default_rng(15)
+    n = 12000
+    mn, mx = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+    pos = torch.from_numpy(rng.uniform(-1, 1, (n, 3)).astype(np.float32)).to(DEV)
+    sup = torch.full((n,), 0.1, dtype=torch.float32, device=DEV)
+    spec = S.CloudSpec(pos, sup, None, None, mn, mx, 0b101, "gather", 4096)
+    plan = S.SearchPlan([spec], want_rows=True)
+    o = oracle.radius_search(pos.cpu().numpy(), sup.cpu().numpy(), domain_min=mn, domain_max=mx,
+                             periodic=[True, False, True], mode="gather", hash_map_length=4096)
+    stream = S._stream()
+    total = ctypes.c_int64(0)
+    check(plan.L.tcr_search_count(*plan._args(), S.ptr(plan.offsets), ctypes.byref(total), stream))
+    P = int(total.value)
+    assert P == o.col.shape[0]
+    for k in range(3):
+        print(f"case: fill {k} after one count", flush=True)
+        row, col = plan._fill_exact(P, stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(col.cpu().numpy(), o.col)
+        np.testing.assert_array_equal(row.cpu().numpy(), o.row)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_repeated_searches_by_dimension(dim):
+    """Repeated searches of one cloud in 1, 2 and 3 dimensions (the later
+    ones on the launch history of the earlier), then moved points; every
+    search against the oracle."""
+    rng = np.random.default_rng(20 + dim)
+    n = {1: 5000, 2: 20000, 3: 30000}[dim]
+    mn, mx = [-1.0] * dim, [1.0] * dim
+    periodic = [True, False, True][:dim]
+    pos = torch.from_numpy(rng.uniform(-1, 1, (n, dim)).astype(np.float32)).to(DEV)
+    h = {1: 0.002, 2: 0.03, 3: 0.1}[dim]   # 1000, 66^2 and 20^3 cells
+    sup = torch.full((n,), h, dtype=torch.float32, device=DEV)
+    dom = _domain(mn, mx, periodic)
+    for step in range(4):
+        print(f"case: repeated dim={dim} step {step}", flush=True)
+        _check(pos, sup, dom, mn, mx, periodic, "csr")
+    pos.copy_(torch.from_numpy(rng.uniform(-1, 1, (n, dim)).astype(np.float32)))
+    print(f"case: repeated dim={dim} moved", flush=True)
+    _check(pos, sup, dom, mn, mx, periodic, "coo")
+
+
+def test_crowded_cell_appears():
+    """Searches of a regular cloud, then of the same cloud with 3000 points
+    gathered into one cell (a cell far over the register chunk)."""
+    rng = np.random.default_rng(24)
+    n = 20000
+    mn, mx, periodic = [-1.0, -1.0], [1.0, 1.0], [True, True]
+    base = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    pos = torch.from_numpy(base).to(DEV)
+    sup = torch.full((n,), 0.05, dtype=torch.float32, device=DEV)
+    dom = _domain(mn, mx, periodic)
+    for step in range(2):
+        print(f"case: crowded regular step {step}", flush=True)
+        _check(pos, sup, dom, mn, mx, periodic, "csr")
+    crowded = base.copy()
+    crowded[:3000] = (np.float32(0.3) + rng.normal(0.0, 1e-4, (3000, 2))).astype(np.float32)
+    pos.copy_(torch.from_numpy(crowded))
+    for step in range(3):
+        print(f"case: crowded cell step {step}", flush=True)
+        _check(pos, sup, dom, mn, mx, periodic, "csr")

@@ -261,6 +261,80 @@ __device__ __forceinline__ void stencil_cell(const GroupArgs &a, const CloudDev 
     for (int o = kOff; o < kStencil; ++o) out[o] = make_int2(0, 0);
 }
 
+// stencil_cell for a cloud with a dense cell index (every reference point in
+// the grid): all lookups of the stencil are issued before any is used --
+// 3^DIM index loads, then the cell ranges, then the bucket checks of the
+// coordinates outside the grid -- instead of one chain of dependent loads
+// per stencil cell.
+template <int DIM>
+__device__ __forceinline__ void stencil_cell_dense(const GroupArgs &a, const CloudDev &cd, const int32_t *q,
+                                                   int2 *__restrict__ out) {
+    constexpr int kOff = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+    int32_t cc[3][3];
+    bool in[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cc[d][k] = 0;
+            in[d][k] = true;
+        }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int32_t n = cd.nc[d];
+        const bool per = (cd.h.periodic_mask >> d) & 1u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t u = q[d] + k - 1;
+            if (per) u = u < 0 ? u + n : (u >= n ? u - n : u);
+            cc[d][k] = u;
+            in[d][k] = u >= 0 && u < n;
+        }
+    }
+    const int32_t *tab = (const int32_t *)(a.lintab + cd.lt_off);
+    const uint32_t P = (uint32_t)grid_cells(cd);  // < 2^31 for a dense index
+    int32_t cid[kOff];
+#pragma unroll
+    for (int o = 0; o < kOff; ++o) {
+        int32_t c[3] = {0, 0, 0};
+        int rem = o;
+#pragma unroll
+        for (int d = DIM - 1; d >= 0; --d) {
+            c[d] = cc[d][rem % 3];
+            rem /= 3;
+        }
+        const uint32_t lin = (uint32_t)linear_index<DIM>(c, cd.nc);
+        const int32_t v = tab[lin < P ? lin : 0u];  // in-bounds load, its value used only when lin < P
+        cid[o] = lin < P ? v : -1;
+    }
+    uint32_t beg[kOff], end[kOff];
+#pragma unroll
+    for (int o = 0; o < kOff; ++o) {
+        const uint32_t k = cid[o] >= 0 ? (uint32_t)cid[o] : 0u;
+        beg[o] = a.cell_begin[k];
+        end[o] = a.cell_begin[k + 1];
+    }
+    const uint32_t L = cd.h.hash_map_length;
+#pragma unroll
+    for (int o = 0; o < kOff; ++o) {
+        int32_t c[3] = {0, 0, 0};
+        bool all_in = true;
+        int rem = o;
+#pragma unroll
+        for (int d = DIM - 1; d >= 0; --d) {
+            const int k = rem % 3;
+            rem /= 3;
+            c[d] = cc[d][k];
+            all_in = all_in && in[d][k];
+        }
+        bool ok = cid[o] >= 0;
+        if (ok && !all_in) ok = a.cell_hash[cid[o]] == hash_index<DIM>(c, L);
+        out[o] = ok ? make_int2((int32_t)beg[o], (int32_t)(end[o] - beg[o])) : make_int2(0, 0);
+    }
+#pragma unroll
+    for (int o = kOff; o < kStencil; ++o) out[o] = make_int2(0, 0);
+}
+
 // One thread per occupied cell: its record and its resolved stencil.
 __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
     const uint32_t ncells = a.misc[kMiscCells];
@@ -291,10 +365,18 @@ __global__ void __launch_bounds__(256) k_stencil(GroupArgs a) {
             for (int o = 0; o < kStencil; ++o) out[o] = make_int2(0, 0);
             continue;
         }
-        switch (dim) {
-            case 1: stencil_cell<1>(a, cd, q, out); break;
-            case 2: stencil_cell<2>(a, cd, q, out); break;
-            default: stencil_cell<3>(a, cd, q, out); break;
+        if (cell_index_dense(cd) && cd.h.nr > 0) {
+            switch (dim) {
+                case 1: stencil_cell_dense<1>(a, cd, q, out); break;
+                case 2: stencil_cell_dense<2>(a, cd, q, out); break;
+                default: stencil_cell_dense<3>(a, cd, q, out); break;
+            }
+        } else {
+            switch (dim) {
+                case 1: stencil_cell<1>(a, cd, q, out); break;
+                case 2: stencil_cell<2>(a, cd, q, out); break;
+                default: stencil_cell<3>(a, cd, q, out); break;
+            }
         }
     }
 }
@@ -332,10 +414,18 @@ __global__ void __launch_bounds__(256) k_qstencil(GroupArgs a) {
             if (((cd.h.periodic_mask >> d) & 1u) && (q[d] == 0 || q[d] == cd.nc[d] - 1)) wrap = true;
         }
         a.recs[cid] = CellRec{gb, ge, c, kCellFast, make_float4(cen[0], cen[1], cen[2], wrap ? 1.0f : 0.0f)};
-        switch (dim) {
-            case 1: stencil_cell<1>(a, cd, q, out); break;
-            case 2: stencil_cell<2>(a, cd, q, out); break;
-            default: stencil_cell<3>(a, cd, q, out); break;
+        if (cell_index_dense(cd) && cd.h.nr > 0) {
+            switch (dim) {
+                case 1: stencil_cell_dense<1>(a, cd, q, out); break;
+                case 2: stencil_cell_dense<2>(a, cd, q, out); break;
+                default: stencil_cell_dense<3>(a, cd, q, out); break;
+            }
+        } else {
+            switch (dim) {
+                case 1: stencil_cell<1>(a, cd, q, out); break;
+                case 2: stencil_cell<2>(a, cd, q, out); break;
+                default: stencil_cell<3>(a, cd, q, out); break;
+            }
         }
     }
 }

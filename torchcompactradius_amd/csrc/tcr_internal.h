@@ -374,6 +374,95 @@ __device__ __forceinline__ uint64_t look_back(const uint64_t *lb, int64_t stride
     }
 }
 
+// Inclusive wave64 scan of v.
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T n = __shfl_up(v, o, kWave);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// Exclusive scan of the THREADS * ITEMS values in lds (in place); returns the
+// block total.  Each thread owns ITEMS consecutive values.
+template <typename T, int THREADS = kScanThreads, int ITEMS = kScanItems>
+__device__ __forceinline__ T block_exclusive_scan_tile(T *lds, T *wave_tot) {
+    const int t = threadIdx.x;
+    T loc[ITEMS];
+    T run = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        loc[k] = lds[t * ITEMS + k];
+        run += loc[k];
+    }
+    T incl = wave_inclusive_scan<T>(run);
+    const int wave = t / kWave, lane = t & (kWave - 1);
+    if (lane == kWave - 1) wave_tot[wave] = incl;
+    __syncthreads();
+    T wave_pre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / kWave; ++w) {
+        const T x = wave_tot[w];
+        if (w < wave) wave_pre += x;
+        total += x;
+    }
+    T ex = wave_pre + incl - run;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        lds[t * ITEMS + k] = ex;
+        ex += loc[k];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Look-back over tiles that publish 62-bit sums (the offsets scan): 2 flag
+// bits over the value.
+constexpr uint64_t kScAgg = 1ull << 62;        // the tile's own sum
+constexpr uint64_t kScInc = 2ull << 62;        // the sum of tiles [0, t]
+constexpr uint64_t kScValue = (1ull << 62) - 1;
+
+// Wave 0 of tile t: publishes the tile's sum, then reads the 64 nearest
+// earlier tiles at once (one lane each; "tile -1" reads as inclusive 0),
+// adds the sums back to the nearest inclusive one and repeats 64 tiles
+// further back while none is inclusive.  Publishes and returns the
+// exclusive prefix.  Tiles take tickets in launch order, so every earlier
+// tile is running and publishes without waiting for this one.
+__device__ __forceinline__ uint64_t tile_look_back(uint64_t *status, int64_t tile, uint64_t agg) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if (tile == 0) {
+        if (lane == 0) lb_store(status, kScInc | agg);
+        return 0;
+    }
+    if (lane == 0) lb_store(status + tile, kScAgg | agg);
+    uint64_t excl = 0;
+    int64_t end = tile - 1;
+    while (true) {
+        const int64_t j = end - lane;
+        const uint64_t v = j >= 0 ? lb_load(status + j) : kScInc;
+        const uint64_t flag = v & ~kScValue;
+        const uint64_t inc = __ballot(flag == kScInc);
+        const uint64_t unready = __ballot(flag == 0);
+        const int k = inc ? __builtin_ctzll(inc) : kWave - 1;  // last lane of this window
+        const uint64_t upto = k == kWave - 1 ? ~0ull : ((2ull << k) - 1);
+        if (unready & upto) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t x = lane <= k ? (v & kScValue) : 0ull;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+        excl += x;
+        if (inc) break;
+        end -= kWave;
+    }
+    if (lane == 0) lb_store(status + tile, kScInc | (excl + agg));
+    return excl;
+}
+
 // lin -> cell lookup with the reference's bucket semantics: the class with
 // linear index `lin` answers a lookup for cell coordinates whose bucket is
 // `bucket` only if the class lives in that bucket (its first point's cell

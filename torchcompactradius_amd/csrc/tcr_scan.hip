@@ -7,51 +7,6 @@
 
 namespace tcr {
 
-// Inclusive wave64 scan of v.
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_scan(T v) {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        T n = __shfl_up(v, o, kWave);
-        if (lane >= o) v += n;
-    }
-    return v;
-}
-
-// Exclusive scan of the kScanTile values in lds (in place); returns the block
-// total.  Each thread owns kScanItems consecutive values.
-template <typename T>
-__device__ __forceinline__ T block_exclusive_scan_tile(T *lds, T *wave_tot) {
-    const int t = threadIdx.x;
-    T loc[kScanItems];
-    T run = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        loc[k] = lds[t * kScanItems + k];
-        run += loc[k];
-    }
-    T incl = wave_inclusive_scan<T>(run);
-    const int wave = t / kWave, lane = t & (kWave - 1);
-    if (lane == kWave - 1) wave_tot[wave] = incl;
-    __syncthreads();
-    T wave_pre = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kScanThreads / kWave; ++w) {
-        const T x = wave_tot[w];
-        if (w < wave) wave_pre += x;
-        total += x;
-    }
-    T ex = wave_pre + incl - run;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        lds[t * kScanItems + k] = ex;
-        ex += loc[k];
-    }
-    __syncthreads();
-    return total;
-}
-
 template <typename Tin>
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const Tin *__restrict__ in, int64_t n,
                                                                uint64_t *__restrict__ part) {
@@ -124,80 +79,43 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const Tin *__restri
     }
 }
 
-// Look-back status of one offsets-scan tile: 2 flag bits over a 62-bit sum.
-constexpr uint64_t kScAgg = 1ull << 62;        // the tile's own sum
-constexpr uint64_t kScInc = 2ull << 62;        // the sum of tiles [0, t]
-constexpr uint64_t kScValue = (1ull << 62) - 1;
-
-// Wave 0 of tile t: publishes the tile's sum, then reads the 64 nearest
-// earlier tiles at once (one lane each; "tile -1" reads as inclusive 0),
-// adds the sums back to the nearest inclusive one and repeats 64 tiles
-// further back while none is inclusive.  Publishes and returns the
-// exclusive prefix.  Every earlier tile holds an earlier ticket, so it is
-// running and publishes without waiting for this one.
-__device__ __forceinline__ uint64_t tile_look_back(uint64_t *status, int64_t tile, uint64_t agg) {
-    const int lane = threadIdx.x & (kWave - 1);
-    if (tile == 0) {
-        if (lane == 0) lb_store(status, kScInc | agg);
-        return 0;
-    }
-    if (lane == 0) lb_store(status + tile, kScAgg | agg);
-    uint64_t excl = 0;
-    int64_t end = tile - 1;
-    while (true) {
-        const int64_t j = end - lane;
-        const uint64_t v = j >= 0 ? lb_load(status + j) : kScInc;
-        const uint64_t flag = v & ~kScValue;
-        const uint64_t inc = __ballot(flag == kScInc);
-        const uint64_t unready = __ballot(flag == 0);
-        const int k = inc ? __builtin_ctzll(inc) : kWave - 1;  // last lane of this window
-        const uint64_t upto = k == kWave - 1 ? ~0ull : ((2ull << k) - 1);
-        if (unready & upto) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        uint64_t x = lane <= k ? (v & kScValue) : 0ull;
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-        excl += x;
-        if (inc) break;
-        end -= kWave;
-    }
-    if (lane == 0) lb_store(status + tile, kScInc | (excl + agg));
-    return excl;
-}
-
 // counts -> int64 offsets in one launch (k_scan_reduce / k_scan_partials /
-// k_scan_apply in one): each tile scans its kScanTile counts in LDS, takes
+// k_scan_apply in one): each tile scans its kOffsetsTile counts in LDS, takes
 // its prefix by look-back, and writes offsets (plus the CSR arrays when
-// asked for); the last tile writes the total.
-__global__ void __launch_bounds__(kScanThreads) k_offsets_scan(const int32_t *__restrict__ in,
-                                                                int64_t *__restrict__ out, int64_t n,
-                                                                uint64_t *__restrict__ status,
-                                                                uint32_t *__restrict__ ticket,
-                                                                int32_t *__restrict__ indptr,
-                                                                int64_t *__restrict__ row_entries) {
-    __shared__ uint64_t lds[kScanTile];
-    __shared__ uint64_t wt[kScanThreads / kWave];
+// asked for); the last tile writes the total.  Tiles of 8192 counts: a
+// million queries make 123 tiles, whose look-back windows of 64 reach an
+// inclusive prefix within two rounds.
+constexpr int kOffsetsThreads = 1024;
+constexpr int kOffsetsItems = 8;
+constexpr int kOffsetsTile = kOffsetsThreads * kOffsetsItems;
+
+__global__ void __launch_bounds__(kOffsetsThreads) k_offsets_scan(const int32_t *__restrict__ in,
+                                                                   int64_t *__restrict__ out, int64_t n,
+                                                                   uint64_t *__restrict__ status,
+                                                                   uint32_t *__restrict__ ticket,
+                                                                   int32_t *__restrict__ indptr,
+                                                                   int64_t *__restrict__ row_entries) {
+    __shared__ uint64_t lds[kOffsetsTile];
+    __shared__ uint64_t wt[kOffsetsThreads / kWave];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_tile;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t base = tile * kScanTile;
-    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    const int64_t base = tile * kOffsetsTile;
+    for (int k = threadIdx.x; k < kOffsetsTile; k += kOffsetsThreads) {
         const int64_t i = base + k;
         lds[k] = (i < n) ? (uint64_t)(int64_t)in[i] : 0;
     }
     __syncthreads();
-    const uint64_t agg = block_exclusive_scan_tile<uint64_t>(lds, wt);
+    const uint64_t agg = block_exclusive_scan_tile<uint64_t, kOffsetsThreads, kOffsetsItems>(lds, wt);
     if (threadIdx.x < kWave) {
         const uint64_t excl = tile_look_back(status, tile, agg);
         if (threadIdx.x == 0) s_excl = excl;
     }
     __syncthreads();
     const uint64_t pre = s_excl;
-    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    for (int k = threadIdx.x; k < kOffsetsTile; k += kOffsetsThreads) {
         const int64_t i = base + k;
         if (i < n) {
             const int64_t o = (int64_t)(lds[k] + pre);
@@ -206,7 +124,7 @@ __global__ void __launch_bounds__(kScanThreads) k_offsets_scan(const int32_t *__
             if (row_entries) row_entries[i] = (int64_t)in[i];
         }
     }
-    if (base + kScanTile >= n && threadIdx.x == 0) {  // the last tile
+    if (base + kOffsetsTile >= n && threadIdx.x == 0) {  // the last tile
         out[n] = (int64_t)(pre + agg);
         if (indptr) indptr[n] = (int32_t)(int64_t)(pre + agg);
     }
@@ -247,7 +165,7 @@ template int exclusive_scan<uint32_t, uint32_t>(const uint32_t *, uint32_t *, in
 template int exclusive_scan<int32_t, int64_t>(const int32_t *, int64_t *, int64_t, bool, uint64_t *,
                                               hipStream_t, int32_t *, int64_t *);
 
-int64_t offsets_scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+int64_t offsets_scan_tiles(int64_t n) { return (n + kOffsetsTile - 1) / kOffsetsTile; }
 
 int offsets_scan(const int32_t *counts, int64_t *offsets, int64_t n, uint64_t *status, uint32_t *ticket,
                  hipStream_t s, int32_t *indptr, int64_t *row_entries) {
@@ -255,7 +173,7 @@ int offsets_scan(const int32_t *counts, int64_t *offsets, int64_t n, uint64_t *s
     if (nb == 0) {
         hipLaunchKernelGGL(k_zero_empty<int64_t>, dim3(1), dim3(1), 0, s, offsets, indptr);
     } else {
-        hipLaunchKernelGGL(k_offsets_scan, dim3((unsigned)nb), dim3(kScanThreads), 0, s, counts, offsets, n, status,
+        hipLaunchKernelGGL(k_offsets_scan, dim3((unsigned)nb), dim3(kOffsetsThreads), 0, s, counts, offsets, n, status,
                            ticket, indptr, row_entries);
     }
     TCR_LAUNCH_CHECK();
