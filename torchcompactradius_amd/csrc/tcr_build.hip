@@ -365,22 +365,16 @@ __global__ void __launch_bounds__(kSortThreads) k_cells(CellsArgs a) {
         if (w < (int)threadIdx.x / kWave) wpre += s_wsum[w];
         ttot += s_wsum[w];
     }
-    // look-back over all earlier tiles of the batch
-    if (threadIdx.x == 0) {
-        uint64_t *lb = a.lb + tile;
-        uint64_t excl = 0;
-        if (tile == 0) {
-            lb_store(lb, kLbInc | ttot);
-        } else {
-            lb_store(lb, kLbAgg | ttot);
-            excl = look_back(a.lb, 1, tile - 1, 0);
-            lb_store(lb, kLbInc | (excl + ttot));
-        }
-        s_excl = excl;
-        if (tile == first_tile) a.desc[c].cell_off = (int64_t)excl;
-        if (tile == a.ntiles - 1) {
-            a.misc[kMiscCells] = (uint32_t)(excl + ttot);
-            a.cell_begin[excl + ttot] = (uint32_t)a.n;
+    // look-back over all earlier tiles of the batch (wave 0, 64 tiles a step)
+    if (threadIdx.x < kWave) {
+        const uint64_t excl = tile_look_back(a.lb, tile, ttot);
+        if (threadIdx.x == 0) {
+            s_excl = excl;
+            if (tile == first_tile) a.desc[c].cell_off = (int64_t)excl;
+            if (tile == a.ntiles - 1) {
+                a.misc[kMiscCells] = (uint32_t)(excl + ttot);
+                a.cell_begin[excl + ttot] = (uint32_t)a.n;
+            }
         }
     }
     __syncthreads();
@@ -637,20 +631,14 @@ __global__ void __launch_bounds__(kSortThreads) k_qcells(const int64_t *__restri
         if (w < (int)threadIdx.x / kWave) wpre += s_wsum[w];
         ttot += s_wsum[w];
     }
-    if (threadIdx.x == 0) {
-        uint64_t *l = lb + tile;
-        uint64_t excl = 0;
-        if (tile == 0) {
-            lb_store(l, kLbInc | ttot);
-        } else {
-            lb_store(l, kLbAgg | ttot);
-            excl = look_back(lb, 1, tile - 1, 0);
-            lb_store(l, kLbInc | (excl + ttot));
-        }
-        s_excl = excl;
-        if (tile == ntiles - 1) {
-            misc[kMiscQCells] = (uint32_t)(excl + ttot);
-            cell_begin[excl + ttot] = (uint32_t)nq;
+    if (threadIdx.x < kWave) {  // look-back: wave 0, 64 tiles a step
+        const uint64_t excl = tile_look_back(lb, tile, ttot);
+        if (threadIdx.x == 0) {
+            s_excl = excl;
+            if (tile == ntiles - 1) {
+                misc[kMiscQCells] = (uint32_t)(excl + ttot);
+                cell_begin[excl + ttot] = (uint32_t)nq;
+            }
         }
     }
     __syncthreads();
