@@ -92,6 +92,8 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     __shared__ uint32_t dbase[kRadix];      // digit base of the cloud in this pass
     __shared__ uint32_t tile_excl[kRadix];  // keys of this digit in earlier tiles of the cloud
     __shared__ uint32_t s_wtot[kRadix / kWave];
+    __shared__ uint32_t s_ttot[kRadix / kWave];
+    __shared__ uint32_t tcount[kRadix];     // the tile's keys per digit, then where each digit starts in the tile
     __shared__ int64_t s_tile;
     const uint32_t npass = a.misc[a.npass_slot];
     if ((uint32_t)a.pass >= npass) return;
@@ -150,6 +152,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
             wcnt[w][d] = run;
             run += c;
         }
+        tcount[d] = run;
         uint64_t *lb = a.lb + tile * kRadix + d;
         if (tile == ti.first) {
             lb_store(lb, kLbInc | run);
@@ -163,44 +166,81 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
         dbase[d] = a.hist[((int64_t)ti.c * kMaxPasses + a.pass) * kRadix + hist_digit(d, a.pass, irr)];
     }
     __syncthreads();
-    // exclusive scan of the cloud's digit counts: wavefront scans of 64
-    // digits, then the wavefront totals
+    // exclusive scans over the digits: of the cloud's digit counts (its
+    // digit bases) and of this tile's (where a digit's keys start in the
+    // tile once ordered by digit): wavefront scans of 64 digits, then the
+    // wavefront totals
     {
         const int d = threadIdx.x;
-        uint32_t incl = 0, c = 0;
+        uint32_t incl = 0, c = 0, tin = 0, tc = 0;
         if (d < kRadix) {
             c = dbase[d];
+            tc = tcount[d];
             incl = c;
+            tin = tc;
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
                 const uint32_t v = __shfl_up(incl, o, kWave);
-                if (lane >= o) incl += v;
+                const uint32_t tv = __shfl_up(tin, o, kWave);
+                if (lane >= o) {
+                    incl += v;
+                    tin += tv;
+                }
             }
-            if (lane == kWave - 1) s_wtot[wave] = incl;
+            if (lane == kWave - 1) {
+                s_wtot[wave] = incl;
+                s_ttot[wave] = tin;
+            }
         }
         __syncthreads();
         if (d < kRadix) {
-            uint32_t pre = 0;
-            for (int w = 0; w < wave; ++w) pre += s_wtot[w];
+            uint32_t pre = 0, tpre = 0;
+            for (int w = 0; w < wave; ++w) {
+                pre += s_wtot[w];
+                tpre += s_ttot[w];
+            }
             dbase[d] = pre + incl - c;
+            tcount[d] = tpre + tin - tc;  // now the tile's digit start
+        }
+    }
+    __syncthreads();
+    // the tile ordered by digit in LDS (over the wave counters, which are
+    // read into registers first), then written out in order: consecutive
+    // lanes store consecutive slots of one digit's run
+    uint32_t lpos[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t d = (eff_key(key[r], irr) >> shift) & (kRadix - 1);
+        lpos[r] = tcount[d] + wcnt[wave][d] + rank[r];
+    }
+    __syncthreads();
+    uint32_t *const s_key = &wcnt[0][0];
+    uint32_t *const s_val = s_key + kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int pos = wave * kPerWave + r * kWave + lane;
+        if (pos < ti.len) {
+            s_key[lpos[r]] = key[r];
+            s_val[lpos[r]] = val[r];
         }
     }
     __syncthreads();
     const int64_t cbase = a.p_start[ti.c];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int pos = wave * kPerWave + r * kWave + lane;
-        if (pos < ti.len) {
-            const uint32_t d = (eff_key(key[r], irr) >> shift) & (kRadix - 1);
-            const int64_t dst = cbase + dbase[d] + tile_excl[d] + wcnt[wave][d] + rank[r];
+        const int i = r * kSortThreads + threadIdx.x;
+        if (i < ti.len) {
+            const uint32_t k = s_key[i], v = s_val[i];
+            const uint32_t d = (eff_key(k, irr) >> shift) & (kRadix - 1);
+            const int64_t dst = cbase + dbase[d] + tile_excl[d] + (i - tcount[d]);
             if (!last) {
-                a.kout[dst] = key[r];
-                a.vout[dst] = val[r];
+                a.kout[dst] = k;
+                a.vout[dst] = v;
             } else {
-                a.sorted[dst] = a.rwrap[cbase + val[r]];
-                a.sort_index[dst] = (int32_t)val[r];
-                a.skey[dst] = (int32_t)key[r];
-                a.sflags[dst] = a.pflags[cbase + val[r]];
+                a.sorted[dst] = a.rwrap[cbase + v];
+                a.sort_index[dst] = (int32_t)v;
+                a.skey[dst] = (int32_t)k;
+                a.sflags[dst] = a.pflags[cbase + v];
             }
         }
     }
