@@ -247,13 +247,15 @@ class SearchPlan:
         check(self.L.tcr_csr_arrays(ptr(self.offsets), self.nq, ptr(indptr), ptr(row_entries), stream))
         return indptr, row_entries
 
-    def run(self, capacity: Optional[int] = None, csr: bool = False) -> "SearchResult":
+    def run(self, capacity: Optional[int] = None, csr: bool = False, stream: Optional[int] = None) -> "SearchResult":
         """One search.  With a capacity guess (default: from the last search
         of this batch shape) count, scan and fill run back to back; a guess
         that is too small costs one extra fill launch.  ``csr`` also returns
         the int32 indptr and int64 rowEntries arrays, computed on the stream
-        before the host waits."""
-        stream = _stream()
+        before the host waits.  ``stream``: the HIP stream handle (default:
+        torch's current stream)."""
+        if stream is None:
+            stream = _stream()
         self._install_tuning()
         ip = re = None
         if capacity is None:
@@ -351,9 +353,10 @@ def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.in
     if len(specs) != 1:
         return SearchPlan(specs, want_rows, index_dtype).run(csr=csr)
     s = specs[0]
+    stream = _stream()
     key = (_tensor_key(s.qpos), _tensor_key(s.qsup), _tensor_key(s.rpos), _tensor_key(s.rsup),
            tuple(s.domain_min), tuple(s.domain_max), s.periodic, s.mode, s.hash_map_length, s.search_range,
-           want_rows, index_dtype, s.qpos.device, _stream())
+           want_rows, index_dtype, s.qpos.device, stream)
     plan = _plan_cache.get(key)
     if plan is None:
         plan = SearchPlan(specs, want_rows, index_dtype)
@@ -362,7 +365,7 @@ def run_search(specs: Sequence[CloudSpec], want_rows: bool, index_dtype=torch.in
             _plan_cache.popitem(last=False)
     else:
         _plan_cache.move_to_end(key)
-    return plan.run(csr=csr)
+    return plan.run(csr=csr, stream=stream)
 
 
 # Keys per query the multi-level count pass may keep (its single-traversal
