@@ -858,7 +858,7 @@ int tcr_mlm_count(const tcr_cloud *cloud, int32_t max_levels, const float *qpos,
     if (rc) return rc;
     rc = mlm_query_launch(p.b, p.w, L, false, nullptr, nullptr, s, slots, slots ? slot_cap : 0);
     if (rc) return rc;
-    rc = exclusive_scan<int32_t, int64_t>(p.w.counts, offsets, p.b.nq_total, true, p.w.scan_part, s);
+    rc = offsets_scan(p.w.counts, offsets, p.b.nq_total, p.w.scan_part, p.w.misc + kMiscScanTicket, s);
     if (rc) return rc;
     Readback *rb = readback_slot();
     if (!rb) {
