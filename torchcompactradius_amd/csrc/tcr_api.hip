@@ -226,10 +226,8 @@ static int prepare(const tcr_cloud *clouds, int32_t n, const float *qpos, const 
         lts[c + 1] = lts[c] + d.lt_cap;
         qts[c + 1] = qts[c] + d.qt_n;
     }
-    if (upload) {
-        std::memcpy(hv.desc, b.dev.data(), sizeof(CloudDev) * b.B);
-        return upload_header(p.header, p.w.desc, s);
-    }
+    std::memcpy(hv.desc, b.dev.data(), sizeof(CloudDev) * b.B);
+    if (upload) return upload_header(p.header, p.w.desc, s);
     return TCR_OK;
 }
 
@@ -349,7 +347,10 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
                          const float *rpos, const float *rsup, void *ws, size_t ws_bytes, int64_t *offsets,
                          PairOut out, int64_t capacity, hipStream_t s, Prepared &p, bool allow_timing = true,
                          CsrOut csr = CsrOut{}, bool spec = false) {
-    int rc = prepare(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, s, p, true);
+    // the header upload is the search's first launch; the host work of the
+    // launch history comes before it, so that the first kernels go out back
+    // to back
+    int rc = prepare(clouds, n, qpos, qsup, rpos, rsup, ws, ws_bytes, s, p, false);
     if (rc) return rc;
     if (!offsets) {
         set_error("offsets missing");
@@ -389,6 +390,8 @@ static int launch_search(const tcr_cloud *clouds, int32_t n, const float *qpos, 
     PhaseEvents pe{};
     const bool timed = g_timing.on && allow_timing;
     if (timed) timing_event(pe, 0, s);
+    rc = upload_header(p.header, p.w.desc, s);
+    if (rc) return rc;
     rc = build_structure(p.b, p.w, s);
     if (rc) return rc;
     KernelEvents &ce = count_events();
