@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(SortPassArgs a) {
     constexpr int kWaves = kSortThreads / kWave;
     constexpr int kPerWave = kSortTile / kWaves;  // 256
     static_assert(kRadix <= kSortThreads, "one thread per digit");
+    static_assert(2 * kSortTile <= kWaves * kRadix, "the tile's keys and values fit the wave counters' LDS");
     __shared__ uint32_t wcnt[kWaves][kRadix];
     __shared__ uint32_t dbase[kRadix];      // digit base of the cloud in this pass
     __shared__ uint32_t tile_excl[kRadix];  // keys of this digit in earlier tiles of the cloud
