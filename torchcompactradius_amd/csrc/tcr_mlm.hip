@@ -465,11 +465,25 @@ template <typename K>
 __device__ __forceinline__ int32_t key_index(K key) {
     return sizeof(K) == 8 ? (int32_t)(uint32_t)key : (int32_t)((uint32_t)key & ((1u << kMlmKey32Shift) - 1u));
 }
+// v of lane (lane ^ j), j a power of two <= 32 known once the sorting
+// network is unrolled: DPP quad permutes for 1 and 2 (no LDS), the swizzle's
+// bit-mask mode for 4, 8 and 16 (no address register), a permute for 32.
+__device__ __forceinline__ uint32_t xor_lane32(uint32_t v, int j) {
+    switch (j) {
+        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (4 << 10) | 0x1F);
+        case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (8 << 10) | 0x1F);
+        case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (16 << 10) | 0x1F);
+        default: return (uint32_t)__shfl_xor((int)v, j, kWave);
+    }
+}
+
 template <typename K>
 __device__ __forceinline__ K shfl_xor_key(K key, int j) {
-    if (sizeof(K) == 4) return (K)(uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
-    const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)key, j, kWave);
-    const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)((uint64_t)key >> 32), j, kWave);
+    if (sizeof(K) == 4) return (K)xor_lane32((uint32_t)key, j);
+    const uint32_t olo = xor_lane32((uint32_t)key, j);
+    const uint32_t ohi = xor_lane32((uint32_t)((uint64_t)key >> 32), j);
     return (K)(((uint64_t)ohi << 32) | olo);
 }
 
